@@ -1,0 +1,152 @@
+#!/usr/bin/env python3
+"""Benchmark of the fluid step on one MI355X.
+
+Workload: the 1024x1024 plain step — 1,048,576 field cells and as many dye
+pixels, 16 random vortices + N(0, 0.05) noise, 64 thresholded Gaussian dye
+blobs (seed 0), K = 8 Jacobi sweeps, dt = 1/120, nu = 0.1.  One "step" is
+the reference's per-timestep work (src/main.cpp:222-225): simulate_fluid_step
+then advect_color_step.  A "particle" of the metric is one field cell plus
+one dye pixel, so value = 1,048,576 x steps / seconds.
+
+Prints one JSON line with the throughput, the roofline of the dominant
+kernel (HIP events on its launch stream, algorithmic bytes per launch) and
+the reference's own CPU code timed on this host on a bounded sample.
+
+    python bench.py [--steps 500] [--warmup 20] [--gpus 1] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s at 1M particles (FP32, %HBM roofline); batched jobs/s + p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(n_cells: int, n_pixels: int, k: int) -> dict:
+    """Minimum HBM bytes of each kernel group, one launch each at K <= 8:
+    advect 16N + K diffusion sweeps 16N + divergence 12N; K pressure sweeps
+    12N + gradient 20N; dye advection 32M + 8N (FP32 u, v, p, b, RGBA)."""
+    return {"diffuse": (28 + 16 * k) * n_cells,
+            "pressure": (20 + 12 * k) * n_cells,
+            "advect_color": 32 * n_pixels + 8 * n_cells}
+
+
+def cpu_baseline(scene, budget_steps: int) -> dict:
+    import oracle
+
+    s = scene.copy()
+    ref = oracle.ref(s.iters)
+    if ref is not None:
+        kind, runner = "reference", lambda: ref.run(s.vp, s.tmp, s.image, s.itmp, s.dt, s.visc,
+                                                     budget_steps)
+        what = "reference src/fluid.cpp (g++ -O2 -ffp-contract=off), 1 thread"
+    else:
+        orc = oracle.get(32)
+        orc.set_threads(1)
+        kind, runner = "port", lambda: orc.run(s.vp, s.tmp, s.image, s.itmp, s.dt, s.visc,
+                                               s.iters, budget_steps)
+        what = "C restatement of src/fluid.cpp (gcc -O2 -ffp-contract=off), 1 thread"
+    t0 = time.perf_counter()
+    runner()
+    dt = time.perf_counter() - t0
+    return {"value": scene.cells * budget_steps / dt, "unit": "particle-steps/s", "cores": 1,
+            "kind": kind,
+            "sample": f"{scene.name}: first {budget_steps} of {scene.steps} steps, {what}, "
+                      f"{dt:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=40,
+                    help="steps of the CPU baseline sample (about 10 s on one core)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(how the step would shard is described in DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from probabilistic_fluid_simulation_amd import fluid, scenes
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        return 1
+    s = scenes.plain_1m(steps=args.steps)
+    vp, tmp, im, itmp = (fluid.VPField.from_numpy(a) for a in (s.vp, s.tmp, s.image, s.itmp))
+
+    def step():
+        fluid.simulate_fluid_step(vp, tmp, s.dt, s.visc, s.iters)
+        fluid.advect_color_step(im, itmp, vp, s.dt)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    value = s.cells * args.steps / wall
+
+    # Per-kernel durations over the same number of steps, with HIP events
+    # recorded on the launch stream around each kernel group.
+    stream = torch.cuda.current_stream()
+    sums = {"diffuse": 0.0, "pressure": 0.0, "advect_color": 0.0}
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for _ in range(args.steps):
+        d_ms, p_ms = fluid.profile_step(vp, tmp, s.dt, s.visc, s.iters, stream=stream)
+        ev0.record(stream)
+        fluid.advect_color_step(im, itmp, vp, s.dt, stream=stream)
+        ev1.record(stream)
+        ev1.synchronize()
+        sums["diffuse"] += d_ms
+        sums["pressure"] += p_ms
+        sums["advect_color"] += ev0.elapsed_time(ev1)
+    avg_ms = {k: v / args.steps for k, v in sums.items()}
+    alg = algorithmic_bytes(s.cells, s.pixels, s.iters)
+    dom = max(avg_ms, key=avg_ms.get)
+    achieved = alg[dom] / (avg_ms[dom] * 1e-3) / 1e9
+    step_bytes = sum(alg.values())
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": s.name, "cells": s.cells, "pixels": s.pixels, "iters": s.iters,
+                   "dt": s.dt, "viscosity": s.visc, "particle": "1 field cell + 1 dye pixel",
+                   "step_algorithmic_bytes": step_bytes,
+                   "step_hbm_roofline_frac": step_bytes * value / s.cells / (HBM_PEAK_GBS * 1e9),
+                   "kernel_avg_ms": avg_ms},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": alg[dom]},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(scenes.plain_1m(steps=args.steps), args.cpu_steps)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

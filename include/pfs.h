@@ -1,0 +1,151 @@
+/*
+ * pfs.h — C ABI of the MI355X (gfx950) fluid-step library libpfs.so.
+ *
+ * The library replaces the reference's solver implementations
+ * (src/fluid.cpp and src/fluid.cu) behind the functions declared in the
+ * reference's includes/fluid.hpp.  Results follow src/fluid.cpp, the CPU
+ * path, bit for bit: same operation order, same buffer ping-pong, same stale
+ * channels, same even/odd Jacobi-count behaviour.
+ *
+ * Conventions
+ *   - Plain C types only.  A field is {x = width, y = height, z = channels,
+ *     data}, row-major interleaved [y][x][z] float32 — the layout of the
+ *     reference's vp_field (includes/fluid.hpp:17-22).  Velocity/pressure
+ *     fields use channels 0 = u, 1 = v, 2 = pressure, 3 = divergence term.
+ *   - Every `data` pointer passed to a pfs_* compute entry point is DEVICE
+ *     memory (hipMalloc or equivalent).  includes/fluid.hpp's host-pointer
+ *     flavour is provided by the C++ header include/fluid.hpp on top of this.
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *     Work is enqueued asynchronously; nothing synchronises unless stated.
+ *   - `iters` is the Jacobi sweep count, the reference's compile-time
+ *     NUM_JACOBI_ITERS (includes/fluid.hpp:11, 30 there), here at run time.
+ *   - Every function returns a status: PFS_OK (0) or a negative PFS_ERR_*;
+ *     pfs_last_error() gives a message for the calling thread.  (The
+ *     reference's functions return void and validate nothing.)
+ *   - Like the reference, functions that ping-pong swap the `data` pointers
+ *     inside the caller's pfs_field structs; the tables below say which.
+ */
+#ifndef PFS_H_
+#define PFS_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PFS_OK 0
+#define PFS_ERR_INVALID (-1)     /* bad argument: null pointer, size, channel count, aliasing */
+#define PFS_ERR_HIP (-2)         /* a HIP runtime call failed */
+#define PFS_ERR_NOMEM (-3)       /* device allocation failed */
+#define PFS_ERR_UNSUPPORTED (-4) /* valid request outside what this build implements */
+
+/* Same member layout as the reference's vp_field (includes/fluid.hpp:17-22). */
+typedef struct pfs_field {
+    int x;       /* width  */
+    int y;       /* height */
+    int z;       /* channels */
+    float *data; /* device pointer, [y][x][z] */
+} pfs_field;
+
+/* ---- library ---------------------------------------------------------- */
+const char *pfs_version(void);
+/* Message of the last failed call on this thread ("" if none). */
+const char *pfs_last_error(void);
+
+/* ---- per-operator entry points ------------------------------------------
+ * One per function of includes/fluid.hpp.  Each computes exactly what the
+ * same-named function of src/fluid.cpp computes. */
+
+/* includes/fluid.hpp:32 / src/fluid.cpp:24-70.  vp_out[0,1] <- bilinear
+ * back-trace of vp[0,1] (periodic wrap); other channels of vp_out untouched.
+ * Needs z >= 2; vp and vp_out must not alias. */
+int pfs_advect(const pfs_field *vp, pfs_field *vp_out, float dt, void *stream);
+
+/* includes/fluid.hpp:45 / src/fluid.cpp:72-127.  out[0..z) <- image advected
+ * by vp's velocity sampled at the nearest cell; image and out same shape. */
+int pfs_advect_color(const pfs_field *image, pfs_field *out, const pfs_field *vp, float dt,
+                     void *stream);
+
+/* includes/fluid.hpp:58 / src/fluid.cpp:129-196.  `iters` Jacobi smoothing
+ * sweeps of channels 0,1 between vp and vp_out; swaps the two structs' data
+ * pointers after every sweep but the last. */
+int pfs_diffuse(pfs_field *vp, pfs_field *vp_out, float viscosity, float dt, int iters,
+                void *stream);
+
+/* includes/fluid.hpp:70 / src/fluid.cpp:198-208: a no-op in the reference. */
+int pfs_add_forces(pfs_field *vp, const float *forces, void *stream);
+
+/* includes/fluid.hpp:81 / src/fluid.cpp:210-267.  Divergence into channel 3
+ * of both fields, then `iters` Jacobi sweeps of channel 2 with the same
+ * pointer swaps as pfs_diffuse.  Needs z >= 4. */
+int pfs_compute_pressure(pfs_field *vp, pfs_field *vp_out, float dt, int iters, void *stream);
+
+/* includes/fluid.hpp:92 / src/fluid.cpp:269-296.  vp_out[0,1] <- vp[0,1]
+ * minus the central-difference gradient of vp[2].  Needs z >= 3. */
+int pfs_subtract_pressure_gradient(const pfs_field *vp, pfs_field *vp_out, float dt,
+                                   void *stream);
+
+/* ---- the step ----------------------------------------------------------- */
+
+/* includes/fluid.hpp:109 / src/fluid.cpp:298-310.  One full step; on return
+ * vp and tmp hold exactly what the reference's CPU step leaves in them (all
+ * four channels of both), and the struct identities are restored as in the
+ * reference.  Uses the fused gfx950 kernels when z == 4 and iters >= 1,
+ * otherwise the per-operator sequence. */
+int pfs_simulate_fluid_step(pfs_field *vp, pfs_field *tmp, float dt, float viscosity, int iters,
+                            void *stream);
+
+/* includes/fluid.hpp:118 / src/fluid.cpp:312-320.  Advect the dye, then swap
+ * image->data and itmp->data. */
+int pfs_advect_color_step(pfs_field *image, pfs_field *itmp, const pfs_field *vp, float dt,
+                          void *stream);
+
+/* includes/fluid.hpp:107.  Device-flavour step on raw device pointers (the
+ * signature src/main.cpp:222 calls, plus iters and stream).  Same results
+ * as pfs_simulate_fluid_step; *vp and *tmp are left un-swapped. */
+int pfs_simulate_fluid_step_dev(float **vp, float **tmp, float dt, float viscosity, int vx,
+                                int vy, int vz, int iters, void *stream);
+
+/* includes/fluid.hpp:116.  Device-flavour dye step (src/main.cpp:225);
+ * swaps *image and *itmp like the reference. */
+int pfs_advect_color_step_dev(float **image, float **itmp, float **vp, float dt, int ix, int iy,
+                              int iz, int vx, int vy, int vz, void *stream);
+
+/* Step-path selection for pfs_simulate_fluid_step / _dev (process-wide).
+ *   PFS_PATH_AUTO     fused kernels when applicable (default)
+ *   PFS_PATH_PER_OP   always the per-operator sequence (the reference's
+ *                     structure, one launch per sweep) */
+#define PFS_PATH_AUTO 0
+#define PFS_PATH_PER_OP 1
+int pfs_set_step_path(int path);
+int pfs_get_step_path(void);
+
+/* Device scratch of the fused step: two float planes of vx*vy cells.  The
+ * library keeps one growable block per device; its size in bytes for a
+ * vx x vy field is returned here. */
+size_t pfs_step_workspace_bytes(int vx, int vy);
+
+/* Per-kernel timing of one fused step on `stream`, measured with HIP events
+ * recorded on that stream around each kernel: ms[0] = advect+diffuse+
+ * divergence kernel, ms[1] = pressure+projection kernel.  Runs one real
+ * step (the buffers advance).  Synchronises the stream. */
+int pfs_profile_step(pfs_field *vp, pfs_field *tmp, float dt, float viscosity, int iters,
+                     void *stream, float *ms, int n_ms);
+
+/* ---- debug: the integer "neighbour lists" ------------------------------
+ * Back-trace cell keys of advect (src/fluid.cpp:55-57): keys[(j*w+i)*4 +
+ * {0,1,2,3}] = {i0, j0, i1, j1} for every cell.  `keys` is a device int32
+ * array of 4*w*h. */
+int pfs_debug_advect_keys(const pfs_field *vp, float dt, int32_t *keys, void *stream);
+/* Keys of advect_color (src/fluid.cpp:89-90,112-113): keys[(j*iw+i)*6 +
+ * {0..5}] = {vi, vj, i0, j0, i1, j1} per pixel; device int32 array of 6*iw*ih. */
+int pfs_debug_advect_color_keys(const pfs_field *image, const pfs_field *vp, float dt,
+                                int32_t *keys, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PFS_H_ */

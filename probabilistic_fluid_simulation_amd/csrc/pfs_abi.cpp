@@ -1,0 +1,276 @@
+// The extern "C" ABI of libpfs.so (include/pfs.h): argument validation,
+// the reference's buffer ping-pong for the per-operator entry points, the
+// step-path choice and the fused step's device workspace.
+
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <string>
+#include <utility>
+
+#include "pfs.h"
+#include "pfs_internal.h"
+
+namespace pfs {
+
+namespace {
+thread_local std::string g_last_error;
+int g_step_path = PFS_PATH_AUTO;
+
+// One growable workspace per (device, stream): calls on one stream are
+// ordered, so they can share it; calls on different streams must not.
+struct Workspace {
+    float *ptr = nullptr;
+    size_t bytes = 0;
+};
+std::mutex g_ws_mutex;
+std::map<std::pair<int, hipStream_t>, Workspace> g_ws;
+
+int get_workspace(size_t bytes, hipStream_t s, float **out) {
+    int dev = 0;
+    PFS_HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    Workspace &w = g_ws[{dev, s}];
+    if (w.bytes < bytes) {
+        if (w.ptr) {
+            // Work already queued on this stream may still use the old block.
+            PFS_HIP_TRY(hipStreamSynchronize(s));
+            PFS_HIP_TRY(hipFree(w.ptr));
+            w.ptr = nullptr;
+            w.bytes = 0;
+        }
+        void *p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return set_error(PFS_ERR_NOMEM, "workspace allocation of " + std::to_string(bytes) +
+                                                " bytes failed");
+        }
+        w.ptr = static_cast<float *>(p);
+        w.bytes = bytes;
+    }
+    *out = w.ptr;
+    return PFS_OK;
+}
+
+bool same_shape(const pfs_field *a, const pfs_field *b) {
+    return a->x == b->x && a->y == b->y && a->z == b->z;
+}
+}  // namespace
+
+int set_error(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+void clear_error() { g_last_error.clear(); }
+
+int check_launch(const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return set_error(PFS_ERR_HIP, std::string(what) + " launch: " + hipGetErrorString(e));
+    return PFS_OK;
+}
+
+int check_field(const pfs_field *f, const char *name, int min_channels) {
+    if (!f) return set_error(PFS_ERR_INVALID, std::string(name) + " is NULL");
+    if (!f->data) return set_error(PFS_ERR_INVALID, std::string(name) + "->data is NULL");
+    if (f->x < 1 || f->y < 1 || f->z < min_channels)
+        return set_error(PFS_ERR_INVALID, std::string(name) + " has shape " +
+                                              std::to_string(f->x) + "x" + std::to_string(f->y) +
+                                              "x" + std::to_string(f->z) + " (needs >= 1x1x" +
+                                              std::to_string(min_channels) + ")");
+    if ((long long)f->x * f->y * f->z > (long long)INT_MAX)
+        return set_error(PFS_ERR_UNSUPPORTED,
+                         std::string(name) + " holds 2^31 or more floats (32-bit indexing)");
+    return PFS_OK;
+}
+
+}  // namespace pfs
+
+using namespace pfs;
+
+extern "C" {
+
+const char *pfs_version(void) { return "pfs 0.1.0 (gfx950)"; }
+
+const char *pfs_last_error(void) { return g_last_error.c_str(); }
+
+int pfs_advect(const pfs_field *vp, pfs_field *vp_out, float dt, void *stream) {
+    PFS_TRY(check_field(vp, "vp", 2));
+    PFS_TRY(check_field(vp_out, "vp_out", 2));
+    if (!same_shape(vp, vp_out)) return set_error(PFS_ERR_INVALID, "vp_out shape != vp shape");
+    if (vp->data == vp_out->data) return set_error(PFS_ERR_INVALID, "vp and vp_out alias");
+    return launch_advect(vp->data, vp_out->data, vp->x, vp->y, vp->z, dt, (hipStream_t)stream);
+}
+
+int pfs_advect_color(const pfs_field *image, pfs_field *out, const pfs_field *vp, float dt,
+                     void *stream) {
+    PFS_TRY(check_field(image, "image", 1));
+    PFS_TRY(check_field(out, "out", 1));
+    PFS_TRY(check_field(vp, "vp", 2));
+    if (!same_shape(image, out)) return set_error(PFS_ERR_INVALID, "out shape != image shape");
+    if (image->data == out->data) return set_error(PFS_ERR_INVALID, "image and out alias");
+    if (out->data == vp->data) return set_error(PFS_ERR_INVALID, "out and vp alias");
+    return launch_advect_color(image->data, out->data, vp->data, image->x, image->y, image->z,
+                               vp->x, vp->y, vp->z, dt, (hipStream_t)stream);
+}
+
+int pfs_diffuse(pfs_field *vp, pfs_field *vp_out, float viscosity, float dt, int iters,
+                void *stream) {
+    PFS_TRY(check_field(vp, "vp", 2));
+    PFS_TRY(check_field(vp_out, "vp_out", 2));
+    if (!same_shape(vp, vp_out)) return set_error(PFS_ERR_INVALID, "vp_out shape != vp shape");
+    if (vp->data == vp_out->data) return set_error(PFS_ERR_INVALID, "vp and vp_out alias");
+    if (iters < 0) return set_error(PFS_ERR_INVALID, "iters < 0");
+    const float alpha = diffuse_alpha(viscosity, dt);
+    const float beta = diffuse_beta(alpha);
+    for (int it = 0; it < iters; ++it) {
+        PFS_TRY(launch_diffuse_sweep(vp->data, vp_out->data, vp->x, vp->y, vp->z, alpha, beta,
+                                     (hipStream_t)stream));
+        if (it != iters - 1) std::swap(vp->data, vp_out->data);  // src/fluid.cpp:188-194
+    }
+    return PFS_OK;
+}
+
+int pfs_add_forces(pfs_field *vp, const float *forces, void *stream) {
+    (void)forces;
+    (void)stream;
+    PFS_TRY(check_field(vp, "vp", 1));
+    return PFS_OK;  // src/fluid.cpp:198-208 is an empty loop
+}
+
+int pfs_compute_pressure(pfs_field *vp, pfs_field *vp_out, float dt, int iters, void *stream) {
+    PFS_TRY(check_field(vp, "vp", 4));
+    PFS_TRY(check_field(vp_out, "vp_out", 4));
+    if (!same_shape(vp, vp_out)) return set_error(PFS_ERR_INVALID, "vp_out shape != vp shape");
+    if (vp->data == vp_out->data) return set_error(PFS_ERR_INVALID, "vp and vp_out alias");
+    if (iters < 0) return set_error(PFS_ERR_INVALID, "iters < 0");
+    hipStream_t s = (hipStream_t)stream;
+    PFS_TRY(launch_divergence(vp->data, vp_out->data, vp->x, vp->y, vp->z, pressure_gamma(dt), s));
+    for (int it = 0; it < iters; ++it) {
+        PFS_TRY(launch_pressure_sweep(vp->data, vp_out->data, vp->x, vp->y, vp->z, s));
+        if (it != iters - 1) std::swap(vp->data, vp_out->data);  // src/fluid.cpp:260-265
+    }
+    return PFS_OK;
+}
+
+int pfs_subtract_pressure_gradient(const pfs_field *vp, pfs_field *vp_out, float dt,
+                                   void *stream) {
+    PFS_TRY(check_field(vp, "vp", 3));
+    PFS_TRY(check_field(vp_out, "vp_out", 2));
+    if (vp->x != vp_out->x || vp->y != vp_out->y || vp->z != vp_out->z)
+        return set_error(PFS_ERR_INVALID, "vp_out shape != vp shape");
+    if (vp->data == vp_out->data) return set_error(PFS_ERR_INVALID, "vp and vp_out alias");
+    return launch_subtract_gradient(vp->data, vp_out->data, vp->x, vp->y, vp->z, dt,
+                                    (hipStream_t)stream);
+}
+
+static int step_per_op(pfs_field *vp, pfs_field *tmp, float dt, float viscosity, int iters,
+                       void *stream) {
+    // src/fluid.cpp:298-310, struct swaps included.
+    PFS_TRY(pfs_advect(vp, tmp, dt, stream));
+    PFS_TRY(pfs_diffuse(tmp, vp, viscosity, dt, iters, stream));
+    PFS_TRY(pfs_compute_pressure(vp, tmp, dt, iters, stream));
+    return pfs_subtract_pressure_gradient(tmp, vp, dt, stream);
+}
+
+static int step_checked(pfs_field *vp, pfs_field *tmp, float dt, float viscosity, int iters,
+                        void *stream, hipEvent_t *ev) {
+    PFS_TRY(check_field(vp, "vp", 4));
+    PFS_TRY(check_field(tmp, "tmp", 4));
+    if (!same_shape(vp, tmp)) return set_error(PFS_ERR_INVALID, "tmp shape != vp shape");
+    if (vp->data == tmp->data) return set_error(PFS_ERR_INVALID, "vp and tmp alias");
+    if (iters < 0) return set_error(PFS_ERR_INVALID, "iters < 0");
+    const bool fused = vp->z == 4 && iters >= 1 && g_step_path == PFS_PATH_AUTO;
+    if (!fused) {
+        if (ev) return set_error(PFS_ERR_UNSUPPORTED, "profiling needs the fused path");
+        return step_per_op(vp, tmp, dt, viscosity, iters, stream);
+    }
+    float *ws = nullptr;
+    PFS_TRY(get_workspace(fused_workspace_bytes(vp->x, vp->y), (hipStream_t)stream, &ws));
+    StepArgs a{vp->data, tmp->data, vp->x, vp->y, dt, viscosity, iters};
+    return fused_step(a, ws, (hipStream_t)stream, ev);
+}
+
+int pfs_simulate_fluid_step(pfs_field *vp, pfs_field *tmp, float dt, float viscosity, int iters,
+                            void *stream) {
+    return step_checked(vp, tmp, dt, viscosity, iters, stream, nullptr);
+}
+
+int pfs_advect_color_step(pfs_field *image, pfs_field *itmp, const pfs_field *vp, float dt,
+                          void *stream) {
+    PFS_TRY(pfs_advect_color(image, itmp, vp, dt, stream));
+    std::swap(image->data, itmp->data);  // src/fluid.cpp:317-319
+    return PFS_OK;
+}
+
+int pfs_simulate_fluid_step_dev(float **vp, float **tmp, float dt, float viscosity, int vx,
+                                int vy, int vz, int iters, void *stream) {
+    if (!vp || !tmp) return set_error(PFS_ERR_INVALID, "vp or tmp is NULL");
+    pfs_field a{vx, vy, vz, *vp}, b{vx, vy, vz, *tmp};
+    PFS_TRY(pfs_simulate_fluid_step(&a, &b, dt, viscosity, iters, stream));
+    *vp = a.data;  // identities are restored by the step
+    *tmp = b.data;
+    return PFS_OK;
+}
+
+int pfs_advect_color_step_dev(float **image, float **itmp, float **vp, float dt, int ix, int iy,
+                              int iz, int vx, int vy, int vz, void *stream) {
+    if (!image || !itmp || !vp) return set_error(PFS_ERR_INVALID, "NULL pointer argument");
+    pfs_field im{ix, iy, iz, *image}, it{ix, iy, iz, *itmp}, v{vx, vy, vz, *vp};
+    PFS_TRY(pfs_advect_color_step(&im, &it, &v, dt, stream));
+    *image = im.data;
+    *itmp = it.data;
+    return PFS_OK;
+}
+
+int pfs_set_step_path(int path) {
+    if (path != PFS_PATH_AUTO && path != PFS_PATH_PER_OP)
+        return set_error(PFS_ERR_INVALID, "unknown step path");
+    g_step_path = path;
+    return PFS_OK;
+}
+
+int pfs_get_step_path(void) { return g_step_path; }
+
+size_t pfs_step_workspace_bytes(int vx, int vy) {
+    if (vx < 1 || vy < 1) return 0;
+    return fused_workspace_bytes(vx, vy);
+}
+
+int pfs_profile_step(pfs_field *vp, pfs_field *tmp, float dt, float viscosity, int iters,
+                     void *stream, float *ms, int n_ms) {
+    if (!ms || n_ms < 2) return set_error(PFS_ERR_INVALID, "ms needs room for 2 values");
+    hipEvent_t ev[4] = {};
+    for (auto &e : ev) PFS_HIP_TRY(hipEventCreate(&e));
+    int rc = step_checked(vp, tmp, dt, viscosity, iters, stream, ev);
+    if (rc == PFS_OK) {
+        hipError_t e = hipEventSynchronize(ev[3]);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms[0], ev[0], ev[1]);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms[1], ev[2], ev[3]);
+        if (e != hipSuccess)
+            rc = set_error(PFS_ERR_HIP, std::string("event timing: ") + hipGetErrorString(e));
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    return rc;
+}
+
+int pfs_debug_advect_keys(const pfs_field *vp, float dt, int32_t *keys, void *stream) {
+    PFS_TRY(check_field(vp, "vp", 2));
+    if (!keys) return set_error(PFS_ERR_INVALID, "keys is NULL");
+    return launch_advect_keys(vp->data, keys, vp->x, vp->y, vp->z, dt, (hipStream_t)stream);
+}
+
+int pfs_debug_advect_color_keys(const pfs_field *image, const pfs_field *vp, float dt,
+                                int32_t *keys, void *stream) {
+    PFS_TRY(check_field(image, "image", 1));
+    PFS_TRY(check_field(vp, "vp", 2));
+    if (!keys) return set_error(PFS_ERR_INVALID, "keys is NULL");
+    return launch_advect_color_keys(vp->data, keys, image->x, image->y, vp->x, vp->y, vp->z, dt,
+                                    (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,99 @@
+// Device helpers shared by the fluid-step kernels (gfx950, wave64).
+//
+// Floating-point contract: every expression below keeps the operation order
+// of the reference's CPU solver (src/fluid.cpp); the library is compiled
+// with -ffp-contract=off (no fused multiply-add), IEEE division and
+// denormals preserved, so each kernel reproduces fluid.cpp bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace pfs {
+
+// fmodf(fmodf(x, n) + n, n) for an integer-valued n > 0 (src/fluid.cpp:48-49,
+// 105-106), exactly.  When |x| < n, fmodf(x, n) == x; x + n is rounded once
+// to s in [0, 2n] and fmodf(s, n) is s, s - n (exact by Sterbenz' lemma for
+// s in [n, 2n)) or 0 for s == 2n.  Other inputs take the libm-exact fmodf.
+__device__ __forceinline__ float wrap_coord(float x, float n) {
+    if (fabsf(x) < n) {
+        float s = x + n;
+        s = (s >= n) ? s - n : s;
+        return (s >= n) ? s - n : s;
+    }
+    return fmodf(fmodf(x, n) + n, n);
+}
+
+// x / n, correctly rounded.  For n a power of two the product with 1/n is the
+// same correctly rounded value of the same real number.
+__device__ __forceinline__ float div_n(float x, float n, float inv_n, bool n_pow2) {
+    return n_pow2 ? x * inv_n : x / n;
+}
+
+// Bilinear blend in the reference's exact order (src/fluid.cpp:62-66).
+__device__ __forceinline__ float bilerp(float sx, float sy, float f00, float f10, float f01,
+                                       float f11) {
+    return (1 - sx) * (1 - sy) * f00 + sx * (1 - sy) * f10 + (1 - sx) * sy * f01 +
+           sx * sy * f11;
+}
+
+// Cell keys of one back-trace (src/fluid.cpp:48-58).
+struct Trace {
+    int i0, j0, i1, j1;
+    float sx, sy;
+};
+
+// Wraps (xp, yp) into [0, w) x [0, h) and returns the corner keys and weights.
+// The clamp of i0/j0 is a no-op for every finite input (the wrapped
+// coordinate lies in [0, n)); it only keeps a NaN/Inf velocity, for which
+// the reference indexes memory out of bounds, inside the field.
+__device__ __forceinline__ Trace trace(float xp, float yp, int w, int h, float fw, float fh) {
+    xp = wrap_coord(xp, fw);
+    yp = wrap_coord(yp, fh);
+    Trace t;
+    t.i0 = min(max((int)xp, 0), w - 1);
+    t.j0 = min(max((int)yp, 0), h - 1);
+    t.i1 = (t.i0 + 1 == w) ? 0 : t.i0 + 1;  // (i0 + 1) % w for i0 in [0, w)
+    t.j1 = (t.j0 + 1 == h) ? 0 : t.j0 + 1;
+    t.sx = xp - (float)t.i0;
+    t.sy = yp - (float)t.j0;
+    return t;
+}
+
+// Periodic neighbours, equal to ((i-1) % n + n) % n and (i+1) % n for i in [0, n).
+__device__ __forceinline__ int wrap_minus(int i, int n) { return i == 0 ? n - 1 : i - 1; }
+__device__ __forceinline__ int wrap_plus(int i, int n) { return i + 1 == n ? 0 : i + 1; }
+
+// Diffusion sweep of one channel (src/fluid.cpp:19-21,168-182).
+__device__ __forceinline__ float diffuse_cell(float l, float r, float t, float b, float c,
+                                              float alpha, float beta) {
+    return (alpha * l + alpha * r + alpha * t + alpha * b + 1.0f * c) / beta;
+}
+
+// Pressure sweep (src/fluid.cpp:249-255): jacobi(pL, pR, pT, pB, 1, 4, b).
+__device__ __forceinline__ float pressure_cell(float l, float r, float t, float b, float rhs) {
+    return (l + r + t + b + 1.0f * rhs) / 4.0f;
+}
+
+// Visits every cell of the rectangle [x0, x0+rw) x [y0, y0+rh) once across
+// the NT threads of a block, row-major, with an incremental (x, y) update
+// instead of a division per cell.
+template <int NT, typename F>
+__device__ __forceinline__ void for_rect(int x0, int y0, int rw, int rh, F &&f) {
+    const int n = rw * rh;
+    int idx = (int)threadIdx.x;
+    if (rw <= 0 || idx >= n) return;
+    int ry = idx / rw;
+    int rx = idx - ry * rw;
+    const int dq = NT / rw, dr = NT - dq * rw;
+    for (; idx < n; idx += NT) {
+        f(x0 + rx, y0 + ry);
+        rx += dr;
+        ry += dq;
+        if (rx >= rw) {
+            rx -= rw;
+            ry += 1;
+        }
+    }
+}
+
+}  // namespace pfs

@@ -1,0 +1,247 @@
+// Per-operator kernels: one launch per operator / per Jacobi sweep, one
+// thread per cell, exactly the structure of the reference's src/fluid.cpp
+// (and of the launch sequence of src/fluid.cu, but with fluid.cpp's results).
+// These back the individual includes/fluid.hpp operators and the
+// per-operator step path; the fused step lives in pfs_step.hip.
+//
+// All kernels address the interleaved [y][x][c] float field with 32-bit
+// indices; the host rejects fields of 2^31 or more floats.
+
+#include "pfs_device.h"
+#include "pfs_internal.h"
+
+namespace pfs {
+
+constexpr int kBX = 64;  // one wavefront across x
+constexpr int kBY = 4;
+
+__global__ __launch_bounds__(kBX *kBY) void k_advect(const float *__restrict__ f,
+                                                     float *__restrict__ o, int w, int h, int c,
+                                                     float dt, float fw, float fh, float inv_w,
+                                                     float inv_h, int pow2_w, int pow2_h) {
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    if (i >= w || j >= h) return;
+    const int base = (j * w + i) * c;
+    const float u = f[base], v = f[base + 1];
+    const float xp = (float)i - div_n(dt * u, fw, inv_w, pow2_w);  // src/fluid.cpp:39
+    const float yp = (float)j - div_n(dt * v, fh, inv_h, pow2_h);  // src/fluid.cpp:41
+    const Trace t = trace(xp, yp, w, h, fw, fh);
+    const int a00 = (t.j0 * w + t.i0) * c, a10 = (t.j0 * w + t.i1) * c;
+    const int a01 = (t.j1 * w + t.i0) * c, a11 = (t.j1 * w + t.i1) * c;
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+        o[base + k] = bilerp(t.sx, t.sy, f[a00 + k], f[a10 + k], f[a01 + k], f[a11 + k]);
+}
+
+__global__ __launch_bounds__(kBX *kBY) void k_advect_color(
+    const float *__restrict__ img, float *__restrict__ out, const float *__restrict__ vp, int iw,
+    int ih, int ic, int vw, int vh, int vc, float dt_viw, float dt_vih, float viw, float vih,
+    float fiw, float fih, float inv_iw, float inv_ih, int pow2_iw, int pow2_ih) {
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    if (i >= iw || j >= ih) return;
+    // Nearest velocity cell (src/fluid.cpp:89-90); clamped only against the
+    // float rounding of i*viw reaching vw, which the reference would read
+    // out of bounds.
+    const int vi = min((int)((float)i * viw), vw - 1);
+    const int vj = min((int)((float)j * vih), vh - 1);
+    const int vb = (vj * vw + vi) * vc;
+    const float xp = (float)i - div_n(dt_viw * vp[vb], fiw, inv_iw, pow2_iw);      // :97
+    const float yp = (float)j - div_n(dt_vih * vp[vb + 1], fih, inv_ih, pow2_ih);  // :98
+    const Trace t = trace(xp, yp, iw, ih, fiw, fih);
+    const int base = (j * iw + i) * ic;
+    const int a00 = (t.j0 * iw + t.i0) * ic, a10 = (t.j0 * iw + t.i1) * ic;
+    const int a01 = (t.j1 * iw + t.i0) * ic, a11 = (t.j1 * iw + t.i1) * ic;
+    if (ic == 4) {
+        const float4 f00 = *reinterpret_cast<const float4 *>(img + a00);
+        const float4 f10 = *reinterpret_cast<const float4 *>(img + a10);
+        const float4 f01 = *reinterpret_cast<const float4 *>(img + a01);
+        const float4 f11 = *reinterpret_cast<const float4 *>(img + a11);
+        float4 r;
+        r.x = bilerp(t.sx, t.sy, f00.x, f10.x, f01.x, f11.x);
+        r.y = bilerp(t.sx, t.sy, f00.y, f10.y, f01.y, f11.y);
+        r.z = bilerp(t.sx, t.sy, f00.z, f10.z, f01.z, f11.z);
+        r.w = bilerp(t.sx, t.sy, f00.w, f10.w, f01.w, f11.w);
+        *reinterpret_cast<float4 *>(out + base) = r;
+    } else {
+        for (int k = 0; k < ic; ++k)
+            out[base + k] =
+                bilerp(t.sx, t.sy, img[a00 + k], img[a10 + k], img[a01 + k], img[a11 + k]);
+    }
+}
+
+// One Jacobi smoothing sweep of channels 0,1 (src/fluid.cpp:154-185).
+__global__ __launch_bounds__(kBX *kBY) void k_diffuse_sweep(const float *__restrict__ in,
+                                                            float *__restrict__ out, int w, int h,
+                                                            int c, float alpha, float beta) {
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    if (i >= w || j >= h) return;
+    const int im = wrap_minus(i, w), ip = wrap_plus(i, w);
+    const int jm = wrap_minus(j, h), jp = wrap_plus(j, h);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const float left = in[(j * w + im) * c + k];
+        const float right = in[(j * w + ip) * c + k];
+        const float top = in[(jm * w + i) * c + k];
+        const float bottom = in[(jp * w + i) * c + k];
+        const float u_n = in[(j * w + i) * c + k];
+        out[(j * w + i) * c + k] = diffuse_cell(left, right, top, bottom, u_n, alpha, beta);
+    }
+}
+
+// Divergence into channel 3 of both fields (src/fluid.cpp:221-237).
+__global__ __launch_bounds__(kBX *kBY) void k_divergence(float *__restrict__ in,
+                                                         float *__restrict__ out, int w, int h,
+                                                         int c, float gamma) {
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    if (i >= w || j >= h) return;
+    const int im = wrap_minus(i, w), ip = wrap_plus(i, w);
+    const int jm = wrap_minus(j, h), jp = wrap_plus(j, h);
+    const float uR = in[(j * w + ip) * c + 0] - in[(j * w + im) * c + 0];
+    const float vT = in[(jp * w + i) * c + 1] - in[(jm * w + i) * c + 1];
+    const float b = gamma * (uR + vT);
+    in[(j * w + i) * c + 3] = b;
+    out[(j * w + i) * c + 3] = b;
+}
+
+// One pressure Jacobi sweep of channel 2 (src/fluid.cpp:240-257).
+__global__ __launch_bounds__(kBX *kBY) void k_pressure_sweep(const float *__restrict__ in,
+                                                             float *__restrict__ out, int w,
+                                                             int h, int c) {
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    if (i >= w || j >= h) return;
+    const int im = wrap_minus(i, w), ip = wrap_plus(i, w);
+    const int jm = wrap_minus(j, h), jp = wrap_plus(j, h);
+    const float pL = in[(j * w + im) * c + 2];
+    const float pR = in[(j * w + ip) * c + 2];
+    const float pT = in[(jm * w + i) * c + 2];
+    const float pB = in[(jp * w + i) * c + 2];
+    const float b = in[(j * w + i) * c + 3];
+    out[(j * w + i) * c + 2] = pressure_cell(pL, pR, pT, pB, b);
+}
+
+// Velocity minus pressure gradient (src/fluid.cpp:273-295).
+__global__ __launch_bounds__(kBX *kBY) void k_subtract_gradient(const float *__restrict__ in,
+                                                                float *__restrict__ out, int w,
+                                                                int h, int c, float dt) {
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    if (i >= w || j >= h) return;
+    const int im = wrap_minus(i, w), ip = wrap_plus(i, w);
+    const int jm = wrap_minus(j, h), jp = wrap_plus(j, h);
+    const float pL = in[(j * w + im) * c + 2];
+    const float pR = in[(j * w + ip) * c + 2];
+    const float pT = in[(jm * w + i) * c + 2];
+    const float pB = in[(jp * w + i) * c + 2];
+    const float gradX = (pR - pL) * dt / 2.0f;
+    const float gradY = (pB - pT) * dt / 2.0f;
+    out[(j * w + i) * c + 0] = in[(j * w + i) * c + 0] - gradX;
+    out[(j * w + i) * c + 1] = in[(j * w + i) * c + 1] - gradY;
+}
+
+// Debug dump of advect's back-trace keys (src/fluid.cpp:55-57).
+__global__ __launch_bounds__(kBX *kBY) void k_advect_keys(const float *__restrict__ f,
+                                                          int *__restrict__ keys, int w, int h,
+                                                          int c, float dt, float fw, float fh,
+                                                          float inv_w, float inv_h, int pow2_w,
+                                                          int pow2_h) {
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    if (i >= w || j >= h) return;
+    const int base = (j * w + i) * c;
+    const float xp = (float)i - div_n(dt * f[base], fw, inv_w, pow2_w);
+    const float yp = (float)j - div_n(dt * f[base + 1], fh, inv_h, pow2_h);
+    const Trace t = trace(xp, yp, w, h, fw, fh);
+    *reinterpret_cast<int4 *>(keys + 4 * (j * w + i)) = make_int4(t.i0, t.j0, t.i1, t.j1);
+}
+
+// Debug dump of advect_color's keys (src/fluid.cpp:89-90,112-113).
+__global__ __launch_bounds__(kBX *kBY) void k_advect_color_keys(
+    const float *__restrict__ vp, int *__restrict__ keys, int iw, int ih, int vw, int vh, int vc,
+    float dt_viw, float dt_vih, float viw, float vih, float fiw, float fih, float inv_iw,
+    float inv_ih, int pow2_iw, int pow2_ih) {
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    if (i >= iw || j >= ih) return;
+    const int vi = min((int)((float)i * viw), vw - 1);
+    const int vj = min((int)((float)j * vih), vh - 1);
+    const int vb = (vj * vw + vi) * vc;
+    const float xp = (float)i - div_n(dt_viw * vp[vb], fiw, inv_iw, pow2_iw);
+    const float yp = (float)j - div_n(dt_vih * vp[vb + 1], fih, inv_ih, pow2_ih);
+    const Trace t = trace(xp, yp, iw, ih, fiw, fih);
+    int *k = keys + 6 * (j * iw + i);
+    k[0] = vi;
+    k[1] = vj;
+    k[2] = t.i0;
+    k[3] = t.j0;
+    k[4] = t.i1;
+    k[5] = t.j1;
+}
+
+namespace {
+dim3 grid_for(int w, int h) { return dim3((w + kBX - 1) / kBX, (h + kBY - 1) / kBY); }
+const dim3 kBlock(kBX, kBY);
+}  // namespace
+
+// ---- host launchers -------------------------------------------------------
+
+int launch_advect(const float *f, float *o, int w, int h, int c, float dt, hipStream_t s) {
+    const Pow2 pw(w), ph(h);
+    hipLaunchKernelGGL(k_advect, grid_for(w, h), kBlock, 0, s, f, o, w, h, c, dt, (float)w,
+                       (float)h, pw.inv, ph.inv, pw.is, ph.is);
+    return check_launch("advect");
+}
+
+int launch_advect_color(const float *img, float *out, const float *vp, int iw, int ih, int ic,
+                        int vw, int vh, int vc, float dt, hipStream_t s) {
+    const ColorConsts k(iw, ih, vw, vh, dt);
+    hipLaunchKernelGGL(k_advect_color, grid_for(iw, ih), kBlock, 0, s, img, out, vp, iw, ih, ic,
+                       vw, vh, vc, k.dt_viw, k.dt_vih, k.viw, k.vih, k.fiw, k.fih, k.piw.inv,
+                       k.pih.inv, k.piw.is, k.pih.is);
+    return check_launch("advect_color");
+}
+
+int launch_diffuse_sweep(const float *in, float *out, int w, int h, int c, float alpha,
+                         float beta, hipStream_t s) {
+    hipLaunchKernelGGL(k_diffuse_sweep, grid_for(w, h), kBlock, 0, s, in, out, w, h, c, alpha,
+                       beta);
+    return check_launch("diffuse_sweep");
+}
+
+int launch_divergence(float *in, float *out, int w, int h, int c, float gamma, hipStream_t s) {
+    hipLaunchKernelGGL(k_divergence, grid_for(w, h), kBlock, 0, s, in, out, w, h, c, gamma);
+    return check_launch("divergence");
+}
+
+int launch_pressure_sweep(const float *in, float *out, int w, int h, int c, hipStream_t s) {
+    hipLaunchKernelGGL(k_pressure_sweep, grid_for(w, h), kBlock, 0, s, in, out, w, h, c);
+    return check_launch("pressure_sweep");
+}
+
+int launch_subtract_gradient(const float *in, float *out, int w, int h, int c, float dt,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(k_subtract_gradient, grid_for(w, h), kBlock, 0, s, in, out, w, h, c, dt);
+    return check_launch("subtract_gradient");
+}
+
+int launch_advect_keys(const float *f, int *keys, int w, int h, int c, float dt, hipStream_t s) {
+    const Pow2 pw(w), ph(h);
+    hipLaunchKernelGGL(k_advect_keys, grid_for(w, h), kBlock, 0, s, f, keys, w, h, c, dt,
+                       (float)w, (float)h, pw.inv, ph.inv, pw.is, ph.is);
+    return check_launch("advect_keys");
+}
+
+int launch_advect_color_keys(const float *vp, int *keys, int iw, int ih, int vw, int vh, int vc,
+                             float dt, hipStream_t s) {
+    const ColorConsts k(iw, ih, vw, vh, dt);
+    hipLaunchKernelGGL(k_advect_color_keys, grid_for(iw, ih), kBlock, 0, s, vp, keys, iw, ih, vw,
+                       vh, vc, k.dt_viw, k.dt_vih, k.viw, k.vih, k.fiw, k.fih, k.piw.inv,
+                       k.pih.inv, k.piw.is, k.pih.is);
+    return check_launch("advect_color_keys");
+}
+
+}  // namespace pfs

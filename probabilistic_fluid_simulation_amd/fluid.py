@@ -1,0 +1,190 @@
+"""Python mirror of the reference's solver interface (includes/fluid.hpp).
+
+Same function names and argument meaning as the reference header; the
+arithmetic runs in libpfs.so's gfx950 kernels on device tensors, with
+results identical to the reference's CPU solver src/fluid.cpp.
+
+``VPField`` plays the role of ``vp_field`` (includes/fluid.hpp:17-22): a
+width x height x channels float32 field in row-major interleaved
+``[y][x][c]`` order, here a contiguous CUDA tensor of shape (H, W, C).
+Functions that swap ``vp_field::data`` pointers in the reference
+(``diffuse``, ``computePressure``, ``advect_color_step``) swap the
+``data`` tensors of the VPField objects passed in, exactly as the
+reference does.
+
+Work is enqueued on torch's current stream and returns without waiting.
+Errors raise ``PfsError`` (the reference's functions return void and
+check nothing).
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import _lib
+
+NUM_JACOBI_ITERS = 30  # includes/fluid.hpp:11; every function below takes it as `iters`
+
+
+class VPField:
+    """vp_field: x = width, y = height, z = channels, data = (H, W, C) float32 tensor."""
+
+    __slots__ = ("data",)
+
+    def __init__(self, data: torch.Tensor):
+        if data.dtype != torch.float32 or data.dim() != 3:
+            raise ValueError("VPField data must be a 3-D float32 tensor (H, W, C)")
+        if not data.is_cuda or not data.is_contiguous():
+            raise ValueError("VPField data must be a contiguous CUDA tensor")
+        self.data = data
+
+    @property
+    def x(self) -> int:
+        return self.data.shape[1]
+
+    @property
+    def y(self) -> int:
+        return self.data.shape[0]
+
+    @property
+    def z(self) -> int:
+        return self.data.shape[2]
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray, device: str = "cuda") -> "VPField":
+        return cls(torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(device))
+
+    @classmethod
+    def empty(cls, w: int, h: int, c: int = 4, device: str = "cuda") -> "VPField":
+        return cls(torch.empty((h, w, c), dtype=torch.float32, device=device))
+
+    def numpy(self) -> np.ndarray:
+        return self.data.detach().cpu().numpy()
+
+    def _struct(self) -> _lib.PfsField:
+        return _lib.PfsField(self.x, self.y, self.z, self.data.data_ptr())
+
+
+def _stream(stream: Optional[torch.cuda.Stream]) -> ctypes.c_void_p:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+class _Bound:
+    """pfs_field structs for a set of VPFields; writes swapped pointers back."""
+
+    def __init__(self, *fields: VPField):
+        self.fields = fields
+        self.structs = [f._struct() for f in fields]
+        self.tensors = {f.data.data_ptr(): f.data for f in fields}
+
+    def ptr(self, k: int):
+        return ctypes.byref(self.structs[k])
+
+    def sync_back(self) -> None:
+        for f, s in zip(self.fields, self.structs):
+            f.data = self.tensors[s.data]
+
+
+# --- the operators of includes/fluid.hpp ---------------------------------------
+def advect(vp: VPField, vp_out: VPField, dt: float, stream=None) -> None:
+    """includes/fluid.hpp:32, src/fluid.cpp:24-70."""
+    b = _Bound(vp, vp_out)
+    _lib.call("pfs_advect", b.ptr(0), b.ptr(1), dt, _stream(stream))
+
+
+def advect_color(image: VPField, out: VPField, vp: VPField, dt: float, stream=None) -> None:
+    """includes/fluid.hpp:45, src/fluid.cpp:72-127."""
+    b = _Bound(image, out, vp)
+    _lib.call("pfs_advect_color", b.ptr(0), b.ptr(1), b.ptr(2), dt, _stream(stream))
+
+
+def diffuse(vp: VPField, vp_out: VPField, viscosity: float, dt: float,
+            iters: int = NUM_JACOBI_ITERS, stream=None) -> None:
+    """includes/fluid.hpp:58, src/fluid.cpp:129-196 (swaps vp.data/vp_out.data
+    after every sweep but the last)."""
+    b = _Bound(vp, vp_out)
+    _lib.call("pfs_diffuse", b.ptr(0), b.ptr(1), viscosity, dt, iters, _stream(stream))
+    b.sync_back()
+
+
+def addForces(vp: VPField, forces=None, stream=None) -> None:  # noqa: N802 (reference name)
+    """includes/fluid.hpp:70, src/fluid.cpp:198-208: a no-op."""
+    b = _Bound(vp)
+    _lib.call("pfs_add_forces", b.ptr(0), None, _stream(stream))
+
+
+def computePressure(vp: VPField, vp_out: VPField, dt: float, iters: int = NUM_JACOBI_ITERS,
+                    stream=None) -> None:  # noqa: N802
+    """includes/fluid.hpp:81, src/fluid.cpp:210-267."""
+    b = _Bound(vp, vp_out)
+    _lib.call("pfs_compute_pressure", b.ptr(0), b.ptr(1), dt, iters, _stream(stream))
+    b.sync_back()
+
+
+def subtractPressureGradient(vp: VPField, vp_out: VPField, dt: float,
+                             stream=None) -> None:  # noqa: N802
+    """includes/fluid.hpp:92, src/fluid.cpp:269-296."""
+    b = _Bound(vp, vp_out)
+    _lib.call("pfs_subtract_pressure_gradient", b.ptr(0), b.ptr(1), dt, _stream(stream))
+
+
+def simulate_fluid_step(vp: VPField, tmp: VPField, dt: float, viscosity: float,
+                        iters: int = NUM_JACOBI_ITERS, stream=None) -> None:
+    """includes/fluid.hpp:109, src/fluid.cpp:298-310."""
+    b = _Bound(vp, tmp)
+    _lib.call("pfs_simulate_fluid_step", b.ptr(0), b.ptr(1), dt, viscosity, iters,
+              _stream(stream))
+    b.sync_back()
+
+
+def advect_color_step(image: VPField, itmp: VPField, vp: VPField, dt: float,
+                      stream=None) -> None:
+    """includes/fluid.hpp:118, src/fluid.cpp:312-320 (swaps image.data/itmp.data)."""
+    b = _Bound(image, itmp, vp)
+    _lib.call("pfs_advect_color_step", b.ptr(0), b.ptr(1), b.ptr(2), dt, _stream(stream))
+    b.sync_back()
+
+
+# --- library controls -------------------------------------------------------------
+def set_step_path(path: str) -> None:
+    """"auto" (fused gfx950 kernels) or "per_op" (one launch per operator/sweep)."""
+    code = {"auto": _lib.PFS_PATH_AUTO, "per_op": _lib.PFS_PATH_PER_OP}[path]
+    _lib.call("pfs_set_step_path", code)
+
+
+def get_step_path() -> str:
+    return {_lib.PFS_PATH_AUTO: "auto", _lib.PFS_PATH_PER_OP: "per_op"}[
+        _lib.load().pfs_get_step_path()]
+
+
+def profile_step(vp: VPField, tmp: VPField, dt: float, viscosity: float, iters: int,
+                 stream=None):
+    """One fused step timed per kernel group with HIP events on the launch
+    stream: returns (diffuse-group ms, pressure-group ms)."""
+    ms = (ctypes.c_float * 2)()
+    b = _Bound(vp, tmp)
+    _lib.call("pfs_profile_step", b.ptr(0), b.ptr(1), dt, viscosity, iters, _stream(stream),
+              ms, 2)
+    return float(ms[0]), float(ms[1])
+
+
+def advect_keys(vp: VPField, dt: float, stream=None) -> torch.Tensor:
+    """Back-trace cell keys (i0, j0, i1, j1) of advect for every cell, (H, W, 4) int32."""
+    keys = torch.empty((vp.y, vp.x, 4), dtype=torch.int32, device=vp.data.device)
+    b = _Bound(vp)
+    _lib.call("pfs_debug_advect_keys", b.ptr(0), dt, ctypes.c_void_p(keys.data_ptr()),
+              _stream(stream))
+    return keys
+
+
+def advect_color_keys(image: VPField, vp: VPField, dt: float, stream=None) -> torch.Tensor:
+    """(vi, vj, i0, j0, i1, j1) of advect_color for every pixel, (Hi, Wi, 6) int32."""
+    keys = torch.empty((image.y, image.x, 6), dtype=torch.int32, device=vp.data.device)
+    b = _Bound(image, vp)
+    _lib.call("pfs_debug_advect_color_keys", b.ptr(0), b.ptr(1), dt,
+              ctypes.c_void_p(keys.data_ptr()), _stream(stream))
+    return keys

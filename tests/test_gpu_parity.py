@@ -1,0 +1,230 @@
+"""GPU parity: libpfs.so's kernels against the CPU oracle (a restatement of
+the reference's src/fluid.cpp pinned by golden vectors from fluid.cpp
+itself).  Every comparison is bit-exact: integer keys and float32 results.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from probabilistic_fluid_simulation_amd import fluid, scenes  # noqa: E402
+from probabilistic_fluid_simulation_amd.fluid import VPField  # noqa: E402
+
+from gpu_util import assert_bitexact, gpu_run, oracle_run, to_dev  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+SHAPES = [  # field w, h, dye w, h
+    (1, 1, 1, 1),
+    (5, 1, 3, 7),
+    (8, 8, 8, 8),
+    (17, 13, 23, 19),
+    (64, 48, 96, 64),
+    (100, 70, 50, 35),
+    (130, 67, 130, 67),
+]
+
+
+@pytest.fixture(autouse=True)
+def _path_reset():
+    yield
+    fluid.set_step_path("auto")
+
+
+def _ids(shapes):
+    return [f"{a}x{b}-img{c}x{d}" for a, b, c, d in shapes]
+
+
+# ------------------------------------------------------------- per operator
+@pytest.mark.parametrize("shape", SHAPES, ids=_ids(SHAPES))
+def test_advect(orc, shape):
+    s = scenes.random_scene(*shape, seed=1)
+    want = s.tmp.copy()
+    orc.advect(s.vp.copy(), want, s.dt)
+    vp, tmp = VPField.from_numpy(s.vp), VPField.from_numpy(s.tmp)
+    fluid.advect(vp, tmp, s.dt)
+    assert_bitexact([tmp.numpy(), vp.numpy()], [want, s.vp], ("out", "vp"))
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=_ids(SHAPES))
+def test_advect_color(orc, shape):
+    s = scenes.random_scene(*shape, seed=2)
+    want = np.zeros_like(s.image)
+    orc.advect_color(s.image.copy(), want, s.vp.copy(), s.dt)
+    im, out, vp = VPField.from_numpy(s.image), VPField.from_numpy(s.itmp), VPField.from_numpy(s.vp)
+    fluid.advect_color(im, out, vp, s.dt)
+    assert_bitexact([out.numpy()], [want], ("out",))
+
+
+@pytest.mark.parametrize("channels", [1, 3, 5])
+def test_advect_color_channel_counts(orc, channels):
+    s = scenes.random_scene(19, 11, 27, 14, seed=3, channels=channels)
+    want = np.zeros_like(s.image)
+    orc.advect_color(s.image.copy(), want, s.vp.copy(), s.dt)
+    im, out, vp = VPField.from_numpy(s.image), VPField.from_numpy(s.itmp), VPField.from_numpy(s.vp)
+    fluid.advect_color(im, out, vp, s.dt)
+    assert_bitexact([out.numpy()], [want], ("out",))
+
+
+@pytest.mark.parametrize("k", [0, 1, 2, 3, 6])
+@pytest.mark.parametrize("shape", SHAPES[1:5], ids=_ids(SHAPES[1:5]))
+def test_diffuse_and_pressure_with_swaps(orc, shape, k):
+    s = scenes.random_scene(*shape, seed=4)
+    a, b = s.vp.copy(), s.tmp.copy()
+    wa, wb = orc.diffuse(a, b, s.visc, s.dt, k)
+    swapped = wa is b
+    vp, tmp = VPField.from_numpy(s.vp), VPField.from_numpy(s.tmp)
+    t_vp, t_tmp = vp.data, tmp.data
+    fluid.diffuse(vp, tmp, s.visc, s.dt, k)
+    assert (vp.data is t_tmp) == swapped and (tmp.data is t_vp) == swapped
+    assert_bitexact([vp.numpy(), tmp.numpy()], [wa, wb], ("vp", "vp_out"))
+
+    a, b = s.vp.copy(), s.tmp.copy()
+    wa, wb = orc.compute_pressure(a, b, s.dt, k)
+    vp, tmp = VPField.from_numpy(s.vp), VPField.from_numpy(s.tmp)
+    fluid.computePressure(vp, tmp, s.dt, k)
+    assert_bitexact([vp.numpy(), tmp.numpy()], [wa, wb], ("vp", "vp_out"))
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=_ids(SHAPES))
+def test_subtract_pressure_gradient(orc, shape):
+    s = scenes.random_scene(*shape, seed=5)
+    want = s.tmp.copy()
+    orc.subtract_pressure_gradient(s.vp.copy(), want, s.dt)
+    vp, out = VPField.from_numpy(s.vp), VPField.from_numpy(s.tmp)
+    fluid.subtractPressureGradient(vp, out, s.dt)
+    assert_bitexact([out.numpy()], [want], ("out",))
+
+
+def test_add_forces_is_noop():
+    s = scenes.random_scene(8, 8, 8, 8, seed=6)
+    vp = VPField.from_numpy(s.vp)
+    fluid.addForces(vp, None)
+    torch.cuda.synchronize()
+    assert vp.numpy().tobytes() == s.vp.tobytes()
+
+
+# ------------------------------------------------------------------- full steps
+STEP_KS = [0, 1, 2, 3, 4, 8, 9, 16, 17, 30]
+
+
+@pytest.mark.parametrize("path", ["auto", "per_op"])
+@pytest.mark.parametrize("k", STEP_KS)
+@pytest.mark.parametrize("shape", [SHAPES[2], SHAPES[3], SHAPES[5], SHAPES[6]],
+                         ids=_ids([SHAPES[2], SHAPES[3], SHAPES[5], SHAPES[6]]))
+def test_steps_random_state(orc, shape, k, path):
+    """Every channel of both buffers random (stale channels and warm starts
+    matter), back-traces crossing several cells, several steps."""
+    fluid.set_step_path(path)
+    s = scenes.random_scene(*shape, seed=7 + k, dt=0.8, visc=0.3, iters=k)
+    assert_bitexact(gpu_run(s, 3), oracle_run(orc, s, 3))
+
+
+@pytest.mark.parametrize("shape", [(256, 192, 256, 192), (200, 333, 400, 111)])
+@pytest.mark.parametrize("k", [4, 7, 8, 12])
+def test_steps_multi_tile(orc, shape, k):
+    """Grids spanning several 64x64 tiles with partial edge tiles."""
+    s = scenes.random_scene(*shape, seed=11, dt=0.5, visc=0.15, iters=k, vel_scale=40.0)
+    assert_bitexact(gpu_run(s, 4), oracle_run(orc, s, 4))
+
+
+def test_steps_large_backtraces(orc):
+    """Back-traces hundreds of cells long take the general fmodf path."""
+    s = scenes.random_scene(96, 80, 96, 80, seed=12, dt=1.0, visc=0.2, iters=4,
+                            vel_scale=500.0 * 96)
+    assert_bitexact(gpu_run(s, 2), oracle_run(orc, s, 2))
+
+
+def _gold_cases():
+    with np.load(os.path.join(GOLD, "steps_small.npz")) as z:
+        n = len([k for k in z.files if k.endswith("_meta")])
+        return [{k[len(f"c{i}_"):]: z[k] for k in z.files if k.startswith(f"c{i}_")}
+                for i in range(n)]
+
+
+@pytest.mark.parametrize("case", _gold_cases(), ids=lambda c: f"K{c['meta'][5]}-{c['meta'][0]}x{c['meta'][1]}")
+def test_golden_vectors_on_gpu(case):
+    """The reference fluid.cpp's own outputs, reproduced by the GPU."""
+    k, steps = int(case["meta"][5]), int(case["meta"][6])
+    s = scenes.Scene(vp=case["in_vp"], tmp=case["in_tmp"], image=case["in_image"],
+                     itmp=np.zeros_like(case["in_image"]), dt=float(case["dt"]),
+                     visc=float(case["visc"]), iters=k, steps=steps)
+    want = [case[f"out_{n}"] for n in ("vp", "tmp", "image", "itmp")]
+    assert_bitexact(gpu_run(s, steps), want)
+
+
+def test_plain16k_matches_reference_digests():
+    """128x128, K = 4, 200 steps: SHA-256 of all buffers equal fluid.cpp's."""
+    with open(os.path.join(GOLD, "plain16k_sha256.json")) as f:
+        gold = json.load(f)
+    s = scenes.plain_16k()
+    vp, tmp, im, itmp = to_dev(s)
+    done = 0
+    for n in (1, 2, 10, 200):
+        for _ in range(n - done):
+            fluid.simulate_fluid_step(vp, tmp, s.dt, s.visc, s.iters)
+            fluid.advect_color_step(im, itmp, vp, s.dt)
+        done = n
+        got = {k: hashlib.sha256(f.numpy().tobytes()).hexdigest()
+               for k, f in zip(("vp", "tmp", "image", "itmp"), (vp, tmp, im, itmp))}
+        assert got == gold["after"][str(n)], n
+
+
+def test_plain_1m_bitexact(orc):
+    """The 1024x1024, K = 8 workload: 3 steps bit-exact, plus its cell keys."""
+    s = scenes.plain_1m()
+    assert_bitexact(gpu_run(s, 3), oracle_run(orc, s, 3))
+
+
+def test_stream_and_repeat_determinism(orc):
+    s = scenes.random_scene(150, 90, 150, 90, seed=13, dt=0.7, visc=0.2, iters=8)
+    st = torch.cuda.Stream()
+    with torch.cuda.stream(st):
+        a = gpu_run(s, 3, stream=st)
+    b = gpu_run(s, 3)
+    assert_bitexact(a, b)
+    assert_bitexact(a, oracle_run(orc, s, 3))
+
+
+def test_profile_step_times_and_advances(orc):
+    s = scenes.random_scene(128, 128, 128, 128, seed=14, dt=0.5, visc=0.2, iters=8)
+    vp, tmp, _, _ = to_dev(s)
+    ms = fluid.profile_step(vp, tmp, s.dt, s.visc, s.iters)
+    assert all(m > 0 for m in ms)
+    a = s.copy()
+    orc.simulate_fluid_step(a.vp, a.tmp, s.dt, s.visc, s.iters)
+    assert_bitexact([vp.numpy(), tmp.numpy()], [a.vp, a.tmp], ("vp", "tmp"))
+
+
+# ------------------------------------------------------------------- cell keys
+def _keys_numpy(vp, dt):
+    """Back-trace keys of src/fluid.cpp:39-57 evaluated with numpy float32
+    scalar-exact operations (np.fmod is IEEE fmod)."""
+    h, w = vp.shape[:2]
+    f32 = np.float32
+    i = np.arange(w, dtype=f32)[None, :]
+    j = np.arange(h, dtype=f32)[:, None]
+    xp = i - (f32(dt) * vp[..., 0]) / f32(w)
+    yp = j - (f32(dt) * vp[..., 1]) / f32(h)
+    xp = np.fmod(np.fmod(xp, f32(w)) + f32(w), f32(w))
+    yp = np.fmod(np.fmod(yp, f32(h)) + f32(h), f32(h))
+    i0 = xp.astype(np.int32)
+    j0 = yp.astype(np.int32)
+    return np.stack([i0, j0, (i0 + 1) % w, (j0 + 1) % h], axis=-1)
+
+
+@pytest.mark.parametrize("shape", [(17, 13), (1024, 1024)])
+def test_advect_keys_bitexact(shape):
+    w, h = shape
+    s = scenes.random_scene(w, h, 4, 4, seed=15, vel_scale=7.0 * w) if w < 100 else scenes.plain_1m()
+    vp = VPField.from_numpy(s.vp)
+    got = fluid.advect_keys(vp, s.dt).cpu().numpy()
+    assert np.array_equal(got, _keys_numpy(s.vp, s.dt))
