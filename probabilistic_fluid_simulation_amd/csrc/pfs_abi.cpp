@@ -184,7 +184,8 @@ static int step_checked(pfs_field *vp, pfs_field *tmp, float dt, float viscosity
     if (!same_shape(vp, tmp)) return set_error(PFS_ERR_INVALID, "tmp shape != vp shape");
     if (vp->data == tmp->data) return set_error(PFS_ERR_INVALID, "vp and tmp alias");
     if (iters < 0) return set_error(PFS_ERR_INVALID, "iters < 0");
-    const bool fused = vp->z == 4 && iters >= 1 && g_step_path == PFS_PATH_AUTO;
+    const bool aligned = ((uintptr_t)vp->data % 16 == 0) && ((uintptr_t)tmp->data % 16 == 0);
+    const bool fused = vp->z == 4 && iters >= 1 && aligned && g_step_path == PFS_PATH_AUTO;
     if (!fused) {
         if (ev) return set_error(PFS_ERR_UNSUPPORTED, "profiling needs the fused path");
         return step_per_op(vp, tmp, dt, viscosity, iters, stream);

@@ -37,7 +37,7 @@ __global__ __launch_bounds__(kBX *kBY) void k_advect(const float *__restrict__ f
 __global__ __launch_bounds__(kBX *kBY) void k_advect_color(
     const float *__restrict__ img, float *__restrict__ out, const float *__restrict__ vp, int iw,
     int ih, int ic, int vw, int vh, int vc, float dt_viw, float dt_vih, float viw, float vih,
-    float fiw, float fih, float inv_iw, float inv_ih, int pow2_iw, int pow2_ih) {
+    float fiw, float fih, float inv_iw, float inv_ih, int pow2_iw, int pow2_ih, int vec4) {
     const int i = blockIdx.x * kBX + threadIdx.x;
     const int j = blockIdx.y * kBY + threadIdx.y;
     if (i >= iw || j >= ih) return;
@@ -53,7 +53,7 @@ __global__ __launch_bounds__(kBX *kBY) void k_advect_color(
     const int base = (j * iw + i) * ic;
     const int a00 = (t.j0 * iw + t.i0) * ic, a10 = (t.j0 * iw + t.i1) * ic;
     const int a01 = (t.j1 * iw + t.i0) * ic, a11 = (t.j1 * iw + t.i1) * ic;
-    if (ic == 4) {
+    if (vec4) {
         const float4 f00 = *reinterpret_cast<const float4 *>(img + a00);
         const float4 f10 = *reinterpret_cast<const float4 *>(img + a10);
         const float4 f01 = *reinterpret_cast<const float4 *>(img + a01);
@@ -199,9 +199,11 @@ int launch_advect(const float *f, float *o, int w, int h, int c, float dt, hipSt
 int launch_advect_color(const float *img, float *out, const float *vp, int iw, int ih, int ic,
                         int vw, int vh, int vc, float dt, hipStream_t s) {
     const ColorConsts k(iw, ih, vw, vh, dt);
+    // float4 loads/stores need 4 channels and 16-byte aligned buffers.
+    const int vec4 = ic == 4 && (uintptr_t)img % 16 == 0 && (uintptr_t)out % 16 == 0;
     hipLaunchKernelGGL(k_advect_color, grid_for(iw, ih), kBlock, 0, s, img, out, vp, iw, ih, ic,
                        vw, vh, vc, k.dt_viw, k.dt_vih, k.viw, k.vih, k.fiw, k.fih, k.piw.inv,
-                       k.pih.inv, k.piw.is, k.pih.is);
+                       k.pih.inv, k.piw.is, k.pih.is, vec4);
     return check_launch("advect_color");
 }
 
