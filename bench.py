@@ -30,12 +30,12 @@ HBM_PEAK_GBS = 8000.0
 
 
 def algorithmic_bytes(n_cells: int, n_pixels: int, k: int) -> dict:
-    """Minimum HBM bytes of each kernel group, one launch each at K <= 8:
+    """Minimum HBM bytes of each kernel, one launch each at K <= 8, counted
+    per stage of the reference's step as if each stage streamed FP32 planes:
     advect 16N + K diffusion sweeps 16N + divergence 12N; K pressure sweeps
-    12N + gradient 20N; dye advection 32M + 8N (FP32 u, v, p, b, RGBA)."""
+    12N + gradient 20N + dye advection 32M + 8N (u, v, p, b, RGBA)."""
     return {"diffuse": (28 + 16 * k) * n_cells,
-            "pressure": (20 + 12 * k) * n_cells,
-            "advect_color": 32 * n_pixels + 8 * n_cells}
+            "pressure_dye": (20 + 12 * k) * n_cells + 32 * n_pixels + 8 * n_cells}
 
 
 def cpu_baseline(scene, budget_steps: int) -> dict:
@@ -70,6 +70,9 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=40,
                     help="steps of the CPU baseline sample (about 10 s on one core)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="after the run, one device-to-device copy of this many MiB (a "
+                         "kernel of known traffic for calibrating HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -86,34 +89,38 @@ def main() -> int:
     s = scenes.plain_1m(steps=args.steps)
     vp, tmp, im, itmp = (fluid.VPField.from_numpy(a) for a in (s.vp, s.tmp, s.image, s.itmp))
 
-    def step():
+    def timestep():  # src/main.cpp:222-225 as one call
+        fluid.fluid_timestep(vp, tmp, im, itmp, s.dt, s.visc, s.iters)
+
+    def two_calls():  # the same timestep as the reference driver's two calls
         fluid.simulate_fluid_step(vp, tmp, s.dt, s.visc, s.iters)
         fluid.advect_color_step(im, itmp, vp, s.dt)
 
+    def timed(fn, n):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
     for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+        timestep()
+    wall = timed(timestep, args.steps)
     value = s.cells * args.steps / wall
+    for _ in range(args.warmup):
+        two_calls()
+    wall_two = timed(two_calls, args.steps)
 
     # Per-kernel durations over the same number of steps, with HIP events
-    # recorded on the launch stream around each kernel group.
+    # recorded on the launch stream around each kernel.
     stream = torch.cuda.current_stream()
-    sums = {"diffuse": 0.0, "pressure": 0.0, "advect_color": 0.0}
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    sums = {"diffuse": 0.0, "pressure_dye": 0.0}
     for _ in range(args.steps):
-        d_ms, p_ms = fluid.profile_step(vp, tmp, s.dt, s.visc, s.iters, stream=stream)
-        ev0.record(stream)
-        fluid.advect_color_step(im, itmp, vp, s.dt, stream=stream)
-        ev1.record(stream)
-        ev1.synchronize()
+        d_ms, p_ms = fluid.profile_step(vp, tmp, s.dt, s.visc, s.iters, image=im, itmp=itmp,
+                                        stream=stream)
         sums["diffuse"] += d_ms
-        sums["pressure"] += p_ms
-        sums["advect_color"] += ev0.elapsed_time(ev1)
+        sums["pressure_dye"] += p_ms
     avg_ms = {k: v / args.steps for k, v in sums.items()}
     alg = algorithmic_bytes(s.cells, s.pixels, s.iters)
     dom = max(avg_ms, key=avg_ms.get)
@@ -137,7 +144,9 @@ def main() -> int:
                    "dt": s.dt, "viscosity": s.visc, "particle": "1 field cell + 1 dye pixel",
                    "step_algorithmic_bytes": step_bytes,
                    "step_hbm_roofline_frac": step_bytes * value / s.cells / (HBM_PEAK_GBS * 1e9),
-                   "kernel_avg_ms": avg_ms},
+                   "kernel_avg_ms": avg_ms,
+                   "entry_point": "pfs_fluid_timestep (simulate_fluid_step + advect_color_step)",
+                   "two_call_ms_per_step": wall_two / args.steps * 1e3},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes_per_launch": alg[dom]},
@@ -145,6 +154,13 @@ def main() -> int:
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scenes.plain_1m(steps=args.steps), args.cpu_steps)
     print(json.dumps(out), flush=True)
+    if args.copy_probe_mib > 0:
+        n = args.copy_probe_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
     return 0
 
 

@@ -102,6 +102,14 @@ int pfs_simulate_fluid_step(pfs_field *vp, pfs_field *tmp, float dt, float visco
 int pfs_advect_color_step(pfs_field *image, pfs_field *itmp, const pfs_field *vp, float dt,
                           void *stream);
 
+/* One timestep of the reference driver (src/main.cpp:222-225, CPU branch
+ * :236-239): simulate_fluid_step(vp, tmp) then advect_color_step(image,
+ * itmp, vp), with the same results and the same image/itmp swap as the two
+ * calls in sequence.  When the dye has the field's size and 4 channels, the
+ * dye advection runs inside the step's last kernel (one pass fewer). */
+int pfs_fluid_timestep(pfs_field *vp, pfs_field *tmp, pfs_field *image, pfs_field *itmp, float dt,
+                       float viscosity, int iters, void *stream);
+
 /* includes/fluid.hpp:107.  Device-flavour step on raw device pointers (the
  * signature src/main.cpp:222 calls, plus iters and stream).  Same results
  * as pfs_simulate_fluid_step; *vp and *tmp are left un-swapped. */
@@ -122,17 +130,20 @@ int pfs_advect_color_step_dev(float **image, float **itmp, float **vp, float dt,
 int pfs_set_step_path(int path);
 int pfs_get_step_path(void);
 
-/* Device scratch of the fused step: two float planes of vx*vy cells.  The
- * library keeps one growable block per device; its size in bytes for a
- * vx x vy field is returned here. */
+/* Device scratch of the fused step: eight float planes of vx*vy cells (the
+ * pressure start, the divergence, and two iterate planes each for diffusion
+ * and pressure when iters exceeds one launch's sweeps).  The library keeps
+ * one growable block per (device, stream); its size in bytes for a vx x vy
+ * field is returned here. */
 size_t pfs_step_workspace_bytes(int vx, int vy);
 
-/* Per-kernel timing of one fused step on `stream`, measured with HIP events
- * recorded on that stream around each kernel: ms[0] = advect+diffuse+
- * divergence kernel, ms[1] = pressure+projection kernel.  Runs one real
+/* Per-kernel timing of one fused step (image/itmp NULL) or timestep
+ * (pfs_fluid_timestep) on `stream`, measured with HIP events recorded on
+ * that stream around each kernel group: ms[0] = advect+diffusion+divergence
+ * kernel(s), ms[1] = pressure+projection(+dye) kernel(s).  Runs one real
  * step (the buffers advance).  Synchronises the stream. */
-int pfs_profile_step(pfs_field *vp, pfs_field *tmp, float dt, float viscosity, int iters,
-                     void *stream, float *ms, int n_ms);
+int pfs_profile_step(pfs_field *vp, pfs_field *tmp, pfs_field *image, pfs_field *itmp, float dt,
+                     float viscosity, int iters, void *stream, float *ms, int n_ms);
 
 /* ---- debug: the integer "neighbour lists" ------------------------------
  * Back-trace cell keys of advect (src/fluid.cpp:55-57): keys[(j*w+i)*4 +

@@ -53,12 +53,13 @@ SIGNATURES = {
     "pfs_subtract_pressure_gradient": [_P, _P, _F, _V],
     "pfs_simulate_fluid_step": [_P, _P, _F, _F, _I, _V],
     "pfs_advect_color_step": [_P, _P, _P, _F, _V],
+    "pfs_fluid_timestep": [_P, _P, _P, _P, _F, _F, _I, _V],
     "pfs_simulate_fluid_step_dev": [_FPP, _FPP, _F, _F, _I, _I, _I, _I, _V],
     "pfs_advect_color_step_dev": [_FPP, _FPP, _FPP, _F, _I, _I, _I, _I, _I, _I, _V],
     "pfs_set_step_path": [_I],
     "pfs_get_step_path": [],
     "pfs_step_workspace_bytes": [_I, _I],
-    "pfs_profile_step": [_P, _P, _F, _F, _I, _V, ctypes.POINTER(ctypes.c_float), _I],
+    "pfs_profile_step": [_P, _P, _P, _P, _F, _F, _I, _V, ctypes.POINTER(ctypes.c_float), _I],
     "pfs_debug_advect_keys": [_P, _F, _V, _V],
     "pfs_debug_advect_color_keys": [_P, _P, _F, _V, _V],
     "pfs_version": [],

@@ -177,28 +177,67 @@ static int step_per_op(pfs_field *vp, pfs_field *tmp, float dt, float viscosity,
     return pfs_subtract_pressure_gradient(tmp, vp, dt, stream);
 }
 
-static int step_checked(pfs_field *vp, pfs_field *tmp, float dt, float viscosity, int iters,
-                        void *stream, hipEvent_t *ev) {
+static bool aligned16(const void *p) { return (uintptr_t)p % 16 == 0; }
+
+static int check_step_fields(const pfs_field *vp, const pfs_field *tmp, int iters) {
     PFS_TRY(check_field(vp, "vp", 4));
     PFS_TRY(check_field(tmp, "tmp", 4));
     if (!same_shape(vp, tmp)) return set_error(PFS_ERR_INVALID, "tmp shape != vp shape");
     if (vp->data == tmp->data) return set_error(PFS_ERR_INVALID, "vp and tmp alias");
     if (iters < 0) return set_error(PFS_ERR_INVALID, "iters < 0");
-    const bool aligned = ((uintptr_t)vp->data % 16 == 0) && ((uintptr_t)tmp->data % 16 == 0);
-    const bool fused = vp->z == 4 && iters >= 1 && aligned && g_step_path == PFS_PATH_AUTO;
-    if (!fused) {
-        if (ev) return set_error(PFS_ERR_UNSUPPORTED, "profiling needs the fused path");
-        return step_per_op(vp, tmp, dt, viscosity, iters, stream);
+    return PFS_OK;
+}
+
+// Whether the fused kernels apply: 4-channel, 16-byte aligned fields, K >= 1.
+static bool fused_ok(const pfs_field *vp, const pfs_field *tmp, int iters) {
+    return g_step_path == PFS_PATH_AUTO && vp->z == 4 && iters >= 1 && aligned16(vp->data) &&
+           aligned16(tmp->data);
+}
+
+// simulate_fluid_step, optionally followed by advect_color_step on
+// (image, itmp) as one timestep of src/main.cpp:222-225.  `ev` (4 events)
+// requests per-kernel timing of the fused path.
+static int timestep(pfs_field *vp, pfs_field *tmp, pfs_field *image, pfs_field *itmp, float dt,
+                    float viscosity, int iters, void *stream, hipEvent_t *ev) {
+    PFS_TRY(check_step_fields(vp, tmp, iters));
+    if (image) {
+        PFS_TRY(check_field(image, "image", 1));
+        PFS_TRY(check_field(itmp, "itmp", 1));
+        if (!same_shape(image, itmp)) return set_error(PFS_ERR_INVALID, "itmp shape != image shape");
+        if (image->data == itmp->data) return set_error(PFS_ERR_INVALID, "image and itmp alias");
+        for (const pfs_field *f : {vp, tmp})
+            if (f->data == image->data || f->data == itmp->data)
+                return set_error(PFS_ERR_INVALID, "dye buffers alias the velocity buffers");
     }
+    if (!fused_ok(vp, tmp, iters)) {
+        if (ev) return set_error(PFS_ERR_UNSUPPORTED, "profiling needs the fused path");
+        PFS_TRY(step_per_op(vp, tmp, dt, viscosity, iters, stream));
+        return image ? pfs_advect_color_step(image, itmp, vp, dt, stream) : PFS_OK;
+    }
+    // The dye rides in the pressure kernel when pixel (i, j) samples cell
+    // (i, j): same size, 4 channels, float4-aligned.
+    const bool dye_fused = image && image->x == vp->x && image->y == vp->y && image->z == 4 &&
+                           aligned16(image->data) && aligned16(itmp->data);
     float *ws = nullptr;
     PFS_TRY(get_workspace(fused_workspace_bytes(vp->x, vp->y), (hipStream_t)stream, &ws));
-    StepArgs a{vp->data, tmp->data, vp->x, vp->y, dt, viscosity, iters};
-    return fused_step(a, ws, (hipStream_t)stream, ev);
+    StepArgs a{vp->data, tmp->data, dye_fused ? image->data : nullptr,
+               dye_fused ? itmp->data : nullptr, vp->x, vp->y, dt, viscosity, iters};
+    PFS_TRY(fused_step(a, ws, (hipStream_t)stream, ev));
+    if (!image) return PFS_OK;
+    if (!dye_fused) return pfs_advect_color_step(image, itmp, vp, dt, stream);
+    std::swap(image->data, itmp->data);  // src/fluid.cpp:317-319
+    return PFS_OK;
 }
 
 int pfs_simulate_fluid_step(pfs_field *vp, pfs_field *tmp, float dt, float viscosity, int iters,
                             void *stream) {
-    return step_checked(vp, tmp, dt, viscosity, iters, stream, nullptr);
+    return timestep(vp, tmp, nullptr, nullptr, dt, viscosity, iters, stream, nullptr);
+}
+
+int pfs_fluid_timestep(pfs_field *vp, pfs_field *tmp, pfs_field *image, pfs_field *itmp, float dt,
+                       float viscosity, int iters, void *stream) {
+    if (!image || !itmp) return set_error(PFS_ERR_INVALID, "image or itmp is NULL");
+    return timestep(vp, tmp, image, itmp, dt, viscosity, iters, stream, nullptr);
 }
 
 int pfs_advect_color_step(pfs_field *image, pfs_field *itmp, const pfs_field *vp, float dt,
@@ -242,12 +281,14 @@ size_t pfs_step_workspace_bytes(int vx, int vy) {
     return fused_workspace_bytes(vx, vy);
 }
 
-int pfs_profile_step(pfs_field *vp, pfs_field *tmp, float dt, float viscosity, int iters,
-                     void *stream, float *ms, int n_ms) {
+int pfs_profile_step(pfs_field *vp, pfs_field *tmp, pfs_field *image, pfs_field *itmp, float dt,
+                     float viscosity, int iters, void *stream, float *ms, int n_ms) {
     if (!ms || n_ms < 2) return set_error(PFS_ERR_INVALID, "ms needs room for 2 values");
+    if ((image == nullptr) != (itmp == nullptr))
+        return set_error(PFS_ERR_INVALID, "image and itmp must both be given or both be NULL");
     hipEvent_t ev[4] = {};
     for (auto &e : ev) PFS_HIP_TRY(hipEventCreate(&e));
-    int rc = step_checked(vp, tmp, dt, viscosity, iters, stream, ev);
+    int rc = timestep(vp, tmp, image, itmp, dt, viscosity, iters, stream, ev);
     if (rc == PFS_OK) {
         hipError_t e = hipEventSynchronize(ev[3]);
         if (e == hipSuccess) e = hipEventElapsedTime(&ms[0], ev[0], ev[1]);

@@ -69,6 +69,35 @@ __device__ __forceinline__ float diffuse_cell(float l, float r, float t, float b
     return (alpha * l + alpha * r + alpha * t + alpha * b + 1.0f * c) / beta;
 }
 
+// x / beta correctly rounded, for a divisor fixed per launch, with y =
+// RN(1/beta) precomputed.  q1 = q0 + (x - q0*beta)*y is within one ulp of
+// x/beta (faithful), so by Markstein's theorem (Markstein 1990; Cornea,
+// Harrison & Tang 2002, final-step theorem) q1 + (x - q1*beta)*y rounded
+// once is RN(x/beta).  The remainders are exact with FMA as long as nothing
+// under- or overflows, which the host guarantees by allowing `fast` only for
+// 2^-20 <= beta <= 2^20 and the range test below keeps for x; every other x
+// (zero, tiny, huge, non-finite) takes IEEE division.
+__device__ __forceinline__ bool div_fast_ok(float x) {
+    const float ax = fabsf(x);
+    return ax >= 0x1p-96f && ax < 0x1p96f;  // false for 0, tiny, huge, Inf, NaN
+}
+// Biased exponent of x: div_fast_ok(x) <=> kDivExpLo <= float_exp(x) <= kDivExpHi.
+// Lets a loop track min/max exponents branch-free and test the range once.
+constexpr unsigned kDivExpLo = 127 - 96, kDivExpHi = 127 + 95;
+__device__ __forceinline__ unsigned float_exp(float x) {
+    return (__float_as_uint(x) >> 23) & 0xffu;
+}
+__device__ __forceinline__ float div_fast(float x, float beta, float y) {
+    const float q0 = x * y;
+    const float r0 = fmaf(-q0, beta, x);
+    const float q1 = fmaf(r0, y, q0);
+    const float r1 = fmaf(-q1, beta, x);
+    return fmaf(r1, y, q1);
+}
+__device__ __forceinline__ float div_fixed(float x, float beta, float y, bool fast) {
+    return (fast && div_fast_ok(x)) ? div_fast(x, beta, y) : x / beta;
+}
+
 // Pressure sweep (src/fluid.cpp:249-255): jacobi(pL, pR, pT, pB, 1, 4, b).
 __device__ __forceinline__ float pressure_cell(float l, float r, float t, float b, float rhs) {
     return (l + r + t + b + 1.0f * rhs) / 4.0f;

@@ -70,8 +70,10 @@ int launch_advect_color_keys(const float *vp, int *keys, int iw, int ih, int vw,
 
 // ---- fused step (pfs_step.hip) -------------------------------------------
 struct StepArgs {
-    float *P;  // vp  (z == 4), device
-    float *Q;  // tmp (z == 4), device
+    float *P;            // vp  (z == 4), device
+    float *Q;            // tmp (z == 4), device
+    const float *image;  // dye (w x h x 4) to advect in the same pass, or null
+    float *itmp;         // its output buffer
     int w, h;
     float dt, visc;
     int iters;

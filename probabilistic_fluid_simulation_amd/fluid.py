@@ -149,6 +149,18 @@ def advect_color_step(image: VPField, itmp: VPField, vp: VPField, dt: float,
     b.sync_back()
 
 
+def fluid_timestep(vp: VPField, tmp: VPField, image: VPField, itmp: VPField, dt: float,
+                   viscosity: float, iters: int = NUM_JACOBI_ITERS, stream=None) -> None:
+    """One timestep of the reference driver loop (src/main.cpp:222-225):
+    simulate_fluid_step then advect_color_step, in one call so that the dye
+    advection can run inside the step's last kernel.  Swaps image.data and
+    itmp.data like advect_color_step."""
+    b = _Bound(vp, tmp, image, itmp)
+    _lib.call("pfs_fluid_timestep", b.ptr(0), b.ptr(1), b.ptr(2), b.ptr(3), dt, viscosity, iters,
+              _stream(stream))
+    b.sync_back()
+
+
 # --- library controls -------------------------------------------------------------
 def set_step_path(path: str) -> None:
     """"auto" (fused gfx950 kernels) or "per_op" (one launch per operator/sweep)."""
@@ -162,13 +174,19 @@ def get_step_path() -> str:
 
 
 def profile_step(vp: VPField, tmp: VPField, dt: float, viscosity: float, iters: int,
-                 stream=None):
-    """One fused step timed per kernel group with HIP events on the launch
-    stream: returns (diffuse-group ms, pressure-group ms)."""
+                 image: Optional[VPField] = None, itmp: Optional[VPField] = None, stream=None):
+    """One fused step (or, with image/itmp, one fluid_timestep) timed per
+    kernel group with HIP events on the launch stream: returns
+    (advect+diffusion+divergence ms, pressure+projection(+dye) ms)."""
     ms = (ctypes.c_float * 2)()
-    b = _Bound(vp, tmp)
-    _lib.call("pfs_profile_step", b.ptr(0), b.ptr(1), dt, viscosity, iters, _stream(stream),
-              ms, 2)
+    if image is None:
+        b = _Bound(vp, tmp)
+        args = (b.ptr(0), b.ptr(1), None, None)
+    else:
+        b = _Bound(vp, tmp, image, itmp)
+        args = (b.ptr(0), b.ptr(1), b.ptr(2), b.ptr(3))
+    _lib.call("pfs_profile_step", *args, dt, viscosity, iters, _stream(stream), ms, 2)
+    b.sync_back()
     return float(ms[0]), float(ms[1])
 
 

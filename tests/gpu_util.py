@@ -13,14 +13,18 @@ def to_dev(scene):
     return tuple(VPField.from_numpy(a) for a in (scene.vp, scene.tmp, scene.image, scene.itmp))
 
 
-def gpu_run(scene, steps, iters=None, stream=None):
-    """steps x (simulate_fluid_step, advect_color_step) on the GPU; returns
-    the four buffers (vp, tmp, image, itmp) as numpy arrays."""
+def gpu_run(scene, steps, iters=None, stream=None, combined=False):
+    """steps x (simulate_fluid_step, advect_color_step) on the GPU — or, with
+    combined=True, steps x fluid_timestep — and return the four buffers
+    (vp, tmp, image, itmp) as numpy arrays."""
     k = scene.iters if iters is None else iters
     vp, tmp, im, itmp = to_dev(scene)
     for _ in range(steps):
-        fluid.simulate_fluid_step(vp, tmp, scene.dt, scene.visc, k, stream=stream)
-        fluid.advect_color_step(im, itmp, vp, scene.dt, stream=stream)
+        if combined:
+            fluid.fluid_timestep(vp, tmp, im, itmp, scene.dt, scene.visc, k, stream=stream)
+        else:
+            fluid.simulate_fluid_step(vp, tmp, scene.dt, scene.visc, k, stream=stream)
+            fluid.advect_color_step(im, itmp, vp, scene.dt, stream=stream)
     torch.cuda.synchronize()
     return tuple(f.numpy() for f in (vp, tmp, im, itmp))
 

@@ -128,6 +128,17 @@ def test_steps_random_state(orc, shape, k, path):
     assert_bitexact(gpu_run(s, 3), oracle_run(orc, s, 3))
 
 
+@pytest.mark.parametrize("k", [1, 2, 3, 8, 9, 17])
+@pytest.mark.parametrize("shape", [(8, 8, 8, 8), (130, 67, 130, 67), (200, 333, 200, 333),
+                                   (64, 48, 96, 64), (100, 70, 50, 35)],
+                         ids=["8x8", "130x67", "200x333", "64x48-img96x64", "100x70-img50x35"])
+def test_fluid_timestep_combined(orc, shape, k):
+    """One-call timestep (dye fused into the last kernel when the dye has the
+    field's size) equals simulate_fluid_step + advect_color_step of fluid.cpp."""
+    s = scenes.random_scene(*shape, seed=21 + k, dt=0.7, visc=0.25, iters=k)
+    assert_bitexact(gpu_run(s, 3, combined=True), oracle_run(orc, s, 3))
+
+
 @pytest.mark.parametrize("shape", [(256, 192, 256, 192), (200, 333, 400, 111)])
 @pytest.mark.parametrize("k", [4, 7, 8, 12])
 def test_steps_multi_tile(orc, shape, k):
@@ -178,10 +189,11 @@ def test_plain16k_matches_reference_digests():
         assert got == gold["after"][str(n)], n
 
 
-def test_plain_1m_bitexact(orc):
-    """The 1024x1024, K = 8 workload: 3 steps bit-exact, plus its cell keys."""
+@pytest.mark.parametrize("combined", [False, True], ids=["two-calls", "timestep"])
+def test_plain_1m_bitexact(orc, combined):
+    """The 1024x1024, K = 8 workload: 3 steps bit-exact."""
     s = scenes.plain_1m()
-    assert_bitexact(gpu_run(s, 3), oracle_run(orc, s, 3))
+    assert_bitexact(gpu_run(s, 3, combined=combined), oracle_run(orc, s, 3))
 
 
 def test_stream_and_repeat_determinism(orc):
@@ -196,12 +208,17 @@ def test_stream_and_repeat_determinism(orc):
 
 def test_profile_step_times_and_advances(orc):
     s = scenes.random_scene(128, 128, 128, 128, seed=14, dt=0.5, visc=0.2, iters=8)
-    vp, tmp, _, _ = to_dev(s)
+    vp, tmp, im, itmp = to_dev(s)
     ms = fluid.profile_step(vp, tmp, s.dt, s.visc, s.iters)
     assert all(m > 0 for m in ms)
     a = s.copy()
     orc.simulate_fluid_step(a.vp, a.tmp, s.dt, s.visc, s.iters)
     assert_bitexact([vp.numpy(), tmp.numpy()], [a.vp, a.tmp], ("vp", "tmp"))
+    ms = fluid.profile_step(vp, tmp, s.dt, s.visc, s.iters, image=im, itmp=itmp)
+    assert all(m > 0 for m in ms)
+    orc.simulate_fluid_step(a.vp, a.tmp, s.dt, s.visc, s.iters)
+    a.image, a.itmp = orc.advect_color_step(a.image, a.itmp, a.vp, s.dt)
+    assert_bitexact([f.numpy() for f in (vp, tmp, im, itmp)], [a.vp, a.tmp, a.image, a.itmp])
 
 
 # ------------------------------------------------------------------- cell keys
