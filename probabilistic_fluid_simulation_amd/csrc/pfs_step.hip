@@ -41,7 +41,7 @@ namespace pfs {
 namespace {
 
 constexpr int kTile = 64;    // tile side
-constexpr int kSegs = 6;     // row segments per staged column
+constexpr int kSegs = 12;    // row segments per staged column
 constexpr int kMaxChunk = 8; // sweeps per launch
 
 template <int H_>
@@ -50,6 +50,7 @@ struct Geo {
     static constexpr int TX = kTile, TY = kTile;
     static constexpr int RX = TX + 2 * H, RY = TY + 2 * H;
     static constexpr int S = (RY + kSegs - 1) / kSegs;  // rows per thread
+    static constexpr int RYP = kSegs * S;               // rows of an LDS plane (>= RY)
     static constexpr int NTH = RX * kSegs;              // threads owning cells
     static constexpr int NT = ((NTH + 63) / 64) * 64;   // whole wavefronts
 };
@@ -78,7 +79,8 @@ struct DiffuseArgs {
     float dt, fw, fh, inv_w, inv_h;
     int pow2_w, pow2_h;
     float alpha, beta, rcp_beta, gamma;
-    int fast_div, first, k_even;
+    int fast_div;  // 0: IEEE division; 1: 5-op exact division; 2: 3-op exact division
+    int first, k_even;
 };
 
 struct PressureArgs {
@@ -97,113 +99,178 @@ struct PressureArgs {
     int first;
 };
 
-// Per-thread view of the staged region.
+// Thread (x, seg) of a tile: column x of the staged region, rows
+// [r0, r0 + S), and the global cells those map to (periodic wrap).
 template <class G>
 struct Strip {
     bool active;
     int x, r0, xl, xr;
-    __device__ explicit Strip(int tid) {
+    int x0, y0;  // global coordinates of region cell (0, 0), unwrapped
+    int gi;      // global column of x
+    int gj[G::S];
+    __device__ Strip(int tid, int tile, int tiles_x, int w, int h) {
         active = tid < G::NTH;
         x = active ? tid % G::RX : 0;
         r0 = active ? (tid / G::RX) * G::S : 0;
         xl = x > 0 ? x - 1 : x;
         xr = x < G::RX - 1 ? x + 1 : x;
+        x0 = (tile % tiles_x) * G::TX - G::H;
+        y0 = (tile / tiles_x) * G::TY - G::H;
+        gi = wrap_index(x0 + x, w);
+        int g = wrap_index(y0 + r0, h);
+#pragma unroll
+        for (int k = 0; k < G::S; ++k) {
+            gj[k] = g;
+            g = (g + 1 == h) ? 0 : g + 1;
+        }
     }
     __device__ bool row_ok(int k) const { return active && r0 + k < G::RY; }
+    // Row r0+k of column x is a tile cell inside the field (not halo or
+    // past the field's edge); then its global cell is (x0+x, y0+r0+k).
+    __device__ bool mine(int k, int w, int h) const {
+        const int r = r0 + k;
+        return active && r >= G::H && r < G::H + G::TY && x >= G::H && x < G::H + G::TX &&
+               x0 + x < w && y0 + r < h;
+    }
+    __device__ int cell(int k, int w) const { return (y0 + r0 + k) * w + x0 + x; }
 };
+
+// Region coordinate of global index g (0 <= g < n) on an axis whose staged
+// region starts at global coordinate o (periodic), or `lim` when g + 1 is
+// not staged right after it.
+__device__ __forceinline__ int staged(int g, int o, int n, int lim) {
+    int r = g - o;  // in (-n, n + H) for o in [-H, n)
+    if (r < 0) r += n;
+    if (r >= n) r -= n;
+    return r < lim - 1 ? r : lim;
+}
 
 template <int H, bool LAST>
 __global__ __launch_bounds__(Geo<H>::NT) void k_diffuse(DiffuseArgs a) {
     using G = Geo<H>;
-    constexpr int S = G::S, RX = G::RX, RY = G::RY;
+    constexpr int S = G::S, RX = G::RX, RY = G::RY, PL = G::RYP * RX;
     constexpr int NS = H - (LAST ? 1 : 0);  // sweeps in this launch
-    __shared__ float2 plane[2][RY * RX];
-    __shared__ int wx[RX], wy[RY];
+    __shared__ float2 plane[2 * PL];
 
-    const int tile = xcd_tile(blockIdx.x, a.ntiles);
-    const int x0 = (tile % a.tiles_x) * G::TX - H, y0 = (tile / a.tiles_x) * G::TY - H;
-    for (int t = threadIdx.x; t < RX; t += G::NT) wx[t] = wrap_index(x0 + t, a.w);
-    for (int t = threadIdx.x; t < RY; t += G::NT) wy[t] = wrap_index(y0 + t, a.h);
-    __syncthreads();
+    const Strip<G> st(threadIdx.x, xcd_tile(blockIdx.x, a.ntiles), a.tiles_x, a.w, a.h);
+    const float2 *__restrict__ P2 = reinterpret_cast<const float2 *>(a.P);  // (u,v) at 2*cell
+    float4 *__restrict__ Q4 = reinterpret_cast<float4 *>(a.Q);
 
-    // Stage the iterate of every region cell in plane 0 (any thread may
-    // produce any cell here), then pick up this thread's strip.
-    {
-        const float2 *P2 = reinterpret_cast<const float2 *>(a.P);  // (u, v) at 2*cell
-        float2 *B0 = plane[0];
-        for_rect<G::NT>(0, 0, RX, RY, [&](int rx, int ry) {
-            const int gi = wx[rx], gj = wy[ry];
-            float2 d;
-            if (a.first) {
-                // Advected velocity of the cell (src/fluid.cpp:36-67).
-                const float2 uv = P2[2 * (gj * a.w + gi)];
-                const float xp = (float)gi - div_n(a.dt * uv.x, a.fw, a.inv_w, a.pow2_w);
-                const float yp = (float)gj - div_n(a.dt * uv.y, a.fh, a.inv_h, a.pow2_h);
-                const Trace t = trace(xp, yp, a.w, a.h, a.fw, a.fh);
-                const float2 f00 = P2[2 * (t.j0 * a.w + t.i0)];
-                const float2 f10 = P2[2 * (t.j0 * a.w + t.i1)];
-                const float2 f01 = P2[2 * (t.j1 * a.w + t.i0)];
-                const float2 f11 = P2[2 * (t.j1 * a.w + t.i1)];
-                d = make_float2(bilerp(t.sx, t.sy, f00.x, f10.x, f01.x, f11.x),
-                                bilerp(t.sx, t.sy, f00.y, f10.y, f01.y, f11.y));
-            } else {
-                d = a.dsrc[gj * a.w + gi];
-            }
-            B0[ry * RX + rx] = d;
-        });
-    }
-    __syncthreads();
-    const Strip<G> st(threadIdx.x);
-    float u[S], v[S];
+    // Every global load of the launch is issued here: the staged region, and
+    // (last chunk) the tile cells' tmp and pressure start, whose latency
+    // then hides behind the advection and the sweeps.
+    float2 c[S];
 #pragma unroll
     for (int k = 0; k < S; ++k) {
-        const float2 d = st.row_ok(k) ? plane[0][(st.r0 + k) * RX + st.x] : make_float2(0.f, 0.f);
-        u[k] = d.x;
-        v[k] = d.y;
+        const int g = st.row_ok(k) ? st.gj[k] * a.w + st.gi : 0;
+        c[k] = a.first ? P2[2 * g] : a.dsrc[g];
+    }
+    float4 q[S];
+    float warm[S];
+    if (LAST) {
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            const bool m = st.mine(k, a.w, a.h);
+            q[k] = m ? Q4[st.cell(k, a.w)] : make_float4(0.f, 0.f, 0.f, 0.f);
+            warm[k] = (m && !a.k_even) ? a.P[4 * st.cell(k, a.w) + 2] : q[k].z;
+        }
+    }
+
+    float u[S], v[S];
+    if (a.first) {
+        // Advect every strip cell (src/fluid.cpp:36-67) with its corners
+        // from the staged velocities in LDS plane 1 — or from memory when a
+        // back-trace leaves the staged region.
+        float2 *V = plane + PL;
+#pragma unroll
+        for (int k = 0; k < S; ++k)
+            if (st.active) V[(st.r0 + k) * RX + st.x] = c[k];
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            const float xp = (float)st.gi - div_n(a.dt * c[k].x, a.fw, a.inv_w, a.pow2_w);
+            const float yp = (float)st.gj[k] - div_n(a.dt * c[k].y, a.fh, a.inv_h, a.pow2_h);
+            const Trace t = trace(xp, yp, a.w, a.h, a.fw, a.fh);
+            const int rx = staged(t.i0, st.x0, a.w, RX), ry = staged(t.j0, st.y0, a.h, RY);
+            float2 f00, f10, f01, f11;
+            if (rx < RX && ry < RY) {
+                const int o = ry * RX + rx;
+                f00 = V[o];
+                f10 = V[o + 1];
+                f01 = V[o + RX];
+                f11 = V[o + RX + 1];
+            } else {
+                f00 = P2[2 * (t.j0 * a.w + t.i0)];
+                f10 = P2[2 * (t.j0 * a.w + t.i1)];
+                f01 = P2[2 * (t.j1 * a.w + t.i0)];
+                f11 = P2[2 * (t.j1 * a.w + t.i1)];
+            }
+            u[k] = bilerp(t.sx, t.sy, f00.x, f10.x, f01.x, f11.x);
+            v[k] = bilerp(t.sx, t.sy, f00.y, f10.y, f01.y, f11.y);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            u[k] = c[k].x;
+            v[k] = c[k].y;
+        }
     }
 
     // Jacobi smoothing sweeps (src/fluid.cpp:154-185):
     // out = (a*L + a*R + a*T + a*B + 1*C) / beta, left to right.  Sweep s
-    // publishes into plane s&1, so plane 0 is free again from sweep 2 on.
-    // The division takes the 5-FMA exactly rounded path; numerators outside
-    // its range are redone with IEEE division in one rare fix-up pass.
+    // publishes alpha-scaled cells into plane (s+1)&1: sweep 1 uses plane 0
+    // while plane 1 may still be read by the advection above.  The division
+    // takes the 5-FMA exactly rounded path; if any numerator of the strip
+    // is outside that path's range (zero, tiny, huge, Inf) the strip is redone
+    // with IEEE division.  Rows past RY are padding: computed, never used.
     float pu[S], pv[S];  // iterate before the last sweep (E for odd K)
     const bool fast = a.fast_div != 0;
     for (int s = 1; s <= NS; ++s) {
-        float2 *B = plane[s & 1];
+        float2 *B = plane + ((s + 1) & 1) * PL;
         float au[S], av[S];
 #pragma unroll
         for (int k = 0; k < S; ++k) {
             au[k] = a.alpha * u[k];
             av[k] = a.alpha * v[k];
-            if (st.row_ok(k)) B[(st.r0 + k) * RX + st.x] = make_float2(au[k], av[k]);
+            if (st.active) B[(st.r0 + k) * RX + st.x] = make_float2(au[k], av[k]);
         }
         __syncthreads();
         if (!st.active) continue;
         const float2 top = B[(st.r0 > 0 ? st.r0 - 1 : 0) * RX + st.x];
         const float2 bot = B[(st.r0 + S < RY ? st.r0 + S : RY - 1) * RX + st.x];
+        const float2 *Bl = B + st.r0 * RX + st.xl, *Br = B + st.r0 * RX + st.xr;
         float nu[S], nv[S];
-        unsigned emin = 0xffu, emax = 0u;  // exponent range of the numerators
+        float lo = 0x1p127f, hi = 0.0f;  // magnitude range of the numerators
+        const bool short_div = a.fast_div == 2;
 #pragma unroll
         for (int k = 0; k < S; ++k) {
             if (LAST && s == NS) {
                 pu[k] = u[k];
                 pv[k] = v[k];
             }
-            const int row = (st.r0 + k < RY ? st.r0 + k : RY - 1) * RX;
-            const float2 L = B[row + st.xl], R = B[row + st.xr];
+            const float2 L = Bl[k * RX], R = Br[k * RX];
             const float tu = k == 0 ? top.x : au[k - 1], tv = k == 0 ? top.y : av[k - 1];
             const float bu = k == S - 1 ? bot.x : au[k + 1];
             const float bv = k == S - 1 ? bot.y : av[k + 1];
             nu[k] = L.x + R.x + tu + bu + 1.0f * u[k];
             nv[k] = L.y + R.y + tv + bv + 1.0f * v[k];
-            u[k] = div_fast(nu[k], a.beta, a.rcp_beta);
-            v[k] = div_fast(nv[k], a.beta, a.rcp_beta);
-            const unsigned eu = float_exp(nu[k]), ev = float_exp(nv[k]);
-            emin = min(emin, min(eu, ev));
-            emax = max(emax, max(eu, ev));
+            lo = fminf(lo, fminf(fabsf(nu[k]), fabsf(nv[k])));
+            hi = fmaxf(hi, fmaxf(fabsf(nu[k]), fabsf(nv[k])));
         }
-        if (!fast || emin < kDivExpLo || emax > kDivExpHi) {
+        if (short_div) {
+#pragma unroll
+            for (int k = 0; k < S; ++k) {
+                u[k] = div_fast3(nu[k], a.beta, a.rcp_beta);
+                v[k] = div_fast3(nv[k], a.beta, a.rcp_beta);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < S; ++k) {
+                u[k] = div_fast(nu[k], a.beta, a.rcp_beta);
+                v[k] = div_fast(nv[k], a.beta, a.rcp_beta);
+            }
+        }
+        if (!fast || !(lo >= 0x1p-96f) || !(hi < 0x1p96f)) {
 #pragma unroll
             for (int k = 0; k < S; ++k) {
                 u[k] = div_fixed(nu[k], a.beta, a.rcp_beta, fast);
@@ -214,149 +281,148 @@ __global__ __launch_bounds__(Geo<H>::NT) void k_diffuse(DiffuseArgs a) {
 
     if (!LAST) {
 #pragma unroll
-        for (int k = 0; k < S; ++k) {
-            const int r = st.r0 + k;
-            if (!st.row_ok(k) || r < H || r >= H + G::TY || st.x < H || st.x >= H + G::TX)
-                continue;
-            const int gx = x0 + st.x, gy = y0 + r;
-            if (gx < a.w && gy < a.h) a.ddst[gy * a.w + gx] = make_float2(u[k], v[k]);
-        }
+        for (int k = 0; k < S; ++k)
+            if (st.mine(k, a.w, a.h)) a.ddst[st.cell(k, a.w)] = make_float2(u[k], v[k]);
         return;
     }
 
-    // Divergence of D^K on the tile (src/fluid.cpp:221-237).
-    float2 *B = plane[(NS + 1) & 1];
+    // Divergence of D^K on the tile (src/fluid.cpp:221-237).  The plane
+    // written here was last read in sweep NS-1, which every thread has left.
+    float2 *B = plane + (NS & 1) * PL;
 #pragma unroll
     for (int k = 0; k < S; ++k)
-        if (st.row_ok(k)) B[(st.r0 + k) * RX + st.x] = make_float2(u[k], v[k]);
+        if (st.active) B[(st.r0 + k) * RX + st.x] = make_float2(u[k], v[k]);
     __syncthreads();
-    float4 *Q4 = reinterpret_cast<float4 *>(a.Q);
 #pragma unroll
     for (int k = 0; k < S; ++k) {
-        const int r = st.r0 + k;
-        if (!st.row_ok(k) || r < H || r >= H + G::TY || st.x < H || st.x >= H + G::TX) continue;
-        const int gx = x0 + st.x, gy = y0 + r;
-        if (gx >= a.w || gy >= a.h) continue;
-        const int c = r * RX + st.x;
-        const float uR = B[c + 1].x - B[c - 1].x;
-        const float vT = B[c + RX].y - B[c - RX].y;
+        if (!st.mine(k, a.w, a.h)) continue;
+        const int o = (st.r0 + k) * RX + st.x;
+        const float uR = B[o + 1].x - B[o - 1].x;
+        const float vT = B[o + RX].y - B[o - RX].y;
         const float b = a.gamma * (uR + vT);
-        const int cell = gy * a.w + gx;
-        const float eu = a.k_even ? u[k] : pu[k];
-        const float ev = a.k_even ? v[k] : pv[k];
-        const float4 q = Q4[cell];
-        a.pw[cell] = a.k_even ? q.z : a.P[4 * cell + 2];
+        const int cell = st.cell(k, a.w);
+        a.pw[cell] = warm[k];
         a.bpl[cell] = b;
-        Q4[cell] = make_float4(eu, ev, q.z, b);
+        Q4[cell] = make_float4(a.k_even ? u[k] : pu[k], a.k_even ? v[k] : pv[k], q[k].z, b);
     }
 }
 
 template <int H, bool LAST, bool DYE>
 __global__ __launch_bounds__(Geo<H>::NT) void k_pressure(PressureArgs a) {
     using G = Geo<H>;
-    constexpr int S = G::S, RX = G::RX, RY = G::RY;
+    constexpr int S = G::S, RX = G::RX, RY = G::RY, PL = G::RYP * RX;
     constexpr int NS = H - (LAST ? 1 : 0);
-    __shared__ float plane[2][RY * RX];
-    __shared__ int wx[RX], wy[RY];
+    __shared__ float plane[2 * PL];
 
-    const int tile = xcd_tile(blockIdx.x, a.ntiles);
-    const int x0 = (tile % a.tiles_x) * G::TX - H, y0 = (tile / a.tiles_x) * G::TY - H;
-    for (int t = threadIdx.x; t < RX; t += G::NT) wx[t] = wrap_index(x0 + t, a.w);
-    for (int t = threadIdx.x; t < RY; t += G::NT) wy[t] = wrap_index(y0 + t, a.h);
-    __syncthreads();
-    const Strip<G> st(threadIdx.x);
+    const Strip<G> st(threadIdx.x, xcd_tile(blockIdx.x, a.ntiles), a.tiles_x, a.w, a.h);
+    float4 *__restrict__ P4 = reinterpret_cast<float4 *>(a.P);
+    float4 *__restrict__ Q4 = reinterpret_cast<float4 *>(a.Q);
 
+    // All global loads up front (see k_diffuse).
     float p[S], rhs[S];
     {
-        const float *src = a.first ? a.pw : a.psrc;
-        const int gi = wx[st.x];
+        const float *__restrict__ src = a.first ? a.pw : a.psrc;
 #pragma unroll
         for (int k = 0; k < S; ++k) {
-            p[k] = 0.0f;
-            rhs[k] = 0.0f;
-            if (!st.row_ok(k)) continue;
-            const int g = wy[st.r0 + k] * a.w + gi;
+            const int g = st.row_ok(k) ? st.gj[k] * a.w + st.gi : 0;
             p[k] = src[g];
             rhs[k] = a.bpl[g];
         }
+    }
+    float4 q[S];  // (E.u, E.v, start, b) of the tile cells
+    if (LAST) {
+#pragma unroll
+        for (int k = 0; k < S; ++k)
+            q[k] = st.mine(k, a.w, a.h) ? Q4[st.cell(k, a.w)] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
 
     // Pressure Jacobi sweeps (src/fluid.cpp:240-257):
     // p' = (pL + pR + pT + pB + 1*b) / 4, left to right.
     float pp[S];  // p^(K-1)
     for (int s = 1; s <= NS; ++s) {
-        float *B = plane[s & 1];
+        float *B = plane + (s & 1) * PL;
 #pragma unroll
         for (int k = 0; k < S; ++k)
-            if (st.row_ok(k)) B[(st.r0 + k) * RX + st.x] = p[k];
+            if (st.active) B[(st.r0 + k) * RX + st.x] = p[k];
         __syncthreads();
-        if (st.active) {
-            const float top = B[(st.r0 > 0 ? st.r0 - 1 : 0) * RX + st.x];
-            const float bot = B[(st.r0 + S < RY ? st.r0 + S : RY - 1) * RX + st.x];
-            float prev = top;
+        if (!st.active) continue;
+        const float top = B[(st.r0 > 0 ? st.r0 - 1 : 0) * RX + st.x];
+        const float bot = B[(st.r0 + S < RY ? st.r0 + S : RY - 1) * RX + st.x];
+        const float *Bl = B + st.r0 * RX + st.xl, *Br = B + st.r0 * RX + st.xr;
+        float prev = top;
 #pragma unroll
-            for (int k = 0; k < S; ++k) {
-                const float cur = p[k];
-                if (LAST && s == NS) pp[k] = cur;
-                if (st.r0 + k < RY) {
-                    const int row = (st.r0 + k) * RX;
-                    const float below = k == S - 1 ? bot : p[k + 1];
-                    p[k] = pressure_cell(B[row + st.xl], B[row + st.xr], prev, below, rhs[k]);
-                }
-                prev = cur;
-            }
+        for (int k = 0; k < S; ++k) {
+            const float cur = p[k];
+            if (LAST && s == NS) pp[k] = cur;
+            const float below = k == S - 1 ? bot : p[k + 1];
+            p[k] = pressure_cell(Bl[k * RX], Br[k * RX], prev, below, rhs[k]);
+            prev = cur;
         }
     }
 
     if (!LAST) {
 #pragma unroll
-        for (int k = 0; k < S; ++k) {
-            const int r = st.r0 + k;
-            if (!st.row_ok(k) || r < H || r >= H + G::TY || st.x < H || st.x >= H + G::TX)
-                continue;
-            const int gx = x0 + st.x, gy = y0 + r;
-            if (gx < a.w && gy < a.h) a.pdst[gy * a.w + gx] = p[k];
-        }
+        for (int k = 0; k < S; ++k)
+            if (st.mine(k, a.w, a.h)) a.pdst[st.cell(k, a.w)] = p[k];
         return;
     }
 
     // Gradient subtraction on the tile (src/fluid.cpp:273-295).
-    float *B = plane[(NS + 1) & 1];
+    float *B = plane + ((NS + 1) & 1) * PL;
 #pragma unroll
     for (int k = 0; k < S; ++k)
-        if (st.row_ok(k)) B[(st.r0 + k) * RX + st.x] = p[k];
+        if (st.active) B[(st.r0 + k) * RX + st.x] = p[k];
     __syncthreads();
-    float4 *P4 = reinterpret_cast<float4 *>(a.P);
-    float4 *Q4 = reinterpret_cast<float4 *>(a.Q);
+    float un[S], vn[S];
 #pragma unroll
     for (int k = 0; k < S; ++k) {
-        const int r = st.r0 + k;
-        if (!st.row_ok(k) || r < H || r >= H + G::TY || st.x < H || st.x >= H + G::TX) continue;
-        const int gx = x0 + st.x, gy = y0 + r;
+        un[k] = vn[k] = 0.0f;
+        if (!st.mine(k, a.w, a.h)) continue;
+        const int o = (st.r0 + k) * RX + st.x;
+        const float gradX = (B[o + 1] - B[o - 1]) * a.dt / 2.0f;
+        const float gradY = (B[o + RX] - B[o - RX]) * a.dt / 2.0f;
+        const int cell = st.cell(k, a.w);
+        un[k] = q[k].x - gradX;
+        vn[k] = q[k].y - gradY;
+        P4[cell] = make_float4(un[k], vn[k], pp[k], rhs[k]);
+        Q4[cell] = make_float4(q[k].x, q[k].y, p[k], rhs[k]);
+    }
+    if (!DYE) return;
+
+    // Dye advection of the tile's pixels (src/fluid.cpp:89-124 with the
+    // image the size of the field, so pixel (i, j) samples cell (i, j)),
+    // pixels spread over all threads so that their gathers are in flight
+    // together.  The new velocities are handed over through LDS: both
+    // planes (>= TX*TY float2) are free once every thread is past the
+    // gradient.
+    static_assert(2 * PL >= 2 * G::TX * G::TY, "velocity hand-over does not fit");
+    __syncthreads();
+    float2 *Vn = reinterpret_cast<float2 *>(plane);
+#pragma unroll
+    for (int k = 0; k < S; ++k)
+        if (st.mine(k, a.w, a.h))
+            Vn[(st.r0 + k - H) * G::TX + (st.x - H)] = make_float2(un[k], vn[k]);
+    __syncthreads();
+    const float4 *__restrict__ I4 = reinterpret_cast<const float4 *>(a.image);
+    float4 *__restrict__ O4 = reinterpret_cast<float4 *>(a.itmp);
+    const int tx0 = st.x0 + H, ty0 = st.y0 + H;
+    constexpr int PER = (G::TX * G::TY + G::NT - 1) / G::NT;
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+        const int idx = threadIdx.x + m * G::NT;
+        if (idx >= G::TX * G::TY) break;
+        const int gx = tx0 + idx % G::TX, gy = ty0 + idx / G::TX;
         if (gx >= a.w || gy >= a.h) continue;
-        const int c = r * RX + st.x;
-        const float gradX = (B[c + 1] - B[c - 1]) * a.dt / 2.0f;
-        const float gradY = (B[c + RX] - B[c - RX]) * a.dt / 2.0f;
-        const int cell = gy * a.w + gx;
-        const float4 q = Q4[cell];  // (E.u, E.v, start, b)
-        const float un = q.x - gradX, vn = q.y - gradY;
-        P4[cell] = make_float4(un, vn, pp[k], rhs[k]);
-        Q4[cell] = make_float4(q.x, q.y, p[k], rhs[k]);
-        if (DYE) {
-            // Dye advection of pixel (gx, gy) through the new velocity of
-            // cell (gx, gy) (src/fluid.cpp:89-124, image size = field size).
-            const float4 *I4 = reinterpret_cast<const float4 *>(a.image);
-            const float xp = (float)gx - div_n(a.dt_viw * un, a.fw, a.inv_w, a.pow2_w);
-            const float yp = (float)gy - div_n(a.dt_vih * vn, a.fh, a.inv_h, a.pow2_h);
-            const Trace t = trace(xp, yp, a.w, a.h, a.fw, a.fh);
-            const float4 f00 = I4[t.j0 * a.w + t.i0], f10 = I4[t.j0 * a.w + t.i1];
-            const float4 f01 = I4[t.j1 * a.w + t.i0], f11 = I4[t.j1 * a.w + t.i1];
-            reinterpret_cast<float4 *>(a.itmp)[cell] =
-                make_float4(bilerp(t.sx, t.sy, f00.x, f10.x, f01.x, f11.x),
-                            bilerp(t.sx, t.sy, f00.y, f10.y, f01.y, f11.y),
-                            bilerp(t.sx, t.sy, f00.z, f10.z, f01.z, f11.z),
-                            bilerp(t.sx, t.sy, f00.w, f10.w, f01.w, f11.w));
-        }
+        const float2 vel = Vn[idx];
+        const float xp = (float)gx - div_n(a.dt_viw * vel.x, a.fw, a.inv_w, a.pow2_w);
+        const float yp = (float)gy - div_n(a.dt_vih * vel.y, a.fh, a.inv_h, a.pow2_h);
+        const Trace t = trace(xp, yp, a.w, a.h, a.fw, a.fh);
+        const float4 f00 = I4[t.j0 * a.w + t.i0], f10 = I4[t.j0 * a.w + t.i1];
+        const float4 f01 = I4[t.j1 * a.w + t.i0], f11 = I4[t.j1 * a.w + t.i1];
+        O4[gy * a.w + gx] = make_float4(bilerp(t.sx, t.sy, f00.x, f10.x, f01.x, f11.x),
+                                        bilerp(t.sx, t.sy, f00.y, f10.y, f01.y, f11.y),
+                                        bilerp(t.sx, t.sy, f00.z, f10.z, f01.z, f11.z),
+                                        bilerp(t.sx, t.sy, f00.w, f10.w, f01.w, f11.w));
     }
 }
 
@@ -456,7 +522,12 @@ int fused_step(const StepArgs &s, float *ws, hipStream_t stream, hipEvent_t *ev)
     d.alpha = alpha;
     d.beta = beta;
     d.rcp_beta = 1.0f / beta;
-    d.fast_div = std::isfinite(beta) && std::fabs(beta) >= 0x1p-20f && std::fabs(beta) <= 0x1p20f;
+    d.fast_div = 0;
+    if (std::isfinite(beta) && beta >= 0x1p-20f && beta <= 0x1p20f) {
+        // y*beta is exact in double (24 x 24 bits).
+        const double e = (double)d.rcp_beta * (double)beta - 1.0;
+        d.fast_div = std::fabs(e) <= 0x1p-26 ? 2 : 1;
+    }
     d.gamma = pressure_gamma(s.dt);
     d.k_even = (K % 2) == 0;
 
