@@ -62,6 +62,72 @@ def cpu_baseline(scene, budget_steps: int) -> dict:
                       f"{dt:.1f} s"}
 
 
+def batched_leg(n_scenes: int, steps: int) -> dict:
+    """The batched workload: n_scenes independent scenes (dye pixels
+    log-uniform in [4096, 262144], field side in {64, 128, 256, 512}, scene s
+    from seed s, 4 Taylor-Green cells + disk dye, K = 8), every scene run for
+    `steps` steps.  Timed twice from identical device-resident inputs: one
+    scene after another with its own launches (pfs_fluid_timestep per step),
+    and all scenes together in one Batch (one launch per kernel stage per
+    step, replayed from a hipGraph).  Both must give bit-identical results."""
+    import torch
+
+    from probabilistic_fluid_simulation_amd import fluid, scenes, serve
+    from probabilistic_fluid_simulation_amd.batch import Batch
+
+    specs = scenes.batched_specs(n_scenes)
+    jobs = [serve.Job(job=sp["scene"], arrival=0.0, fw=sp["fw"], fh=sp["fh"], iw=sp["iw"],
+                      ih=sp["ih"], steps=steps, seed=sp["seed"], iters=8) for sp in specs]
+    inputs = [serve.device_scene(j) for j in jobs]
+    torch.cuda.synchronize()
+
+    def clone(bufs):
+        return [fluid.VPField(b.data.clone()) for b in bufs]
+
+    plain = [clone(b) for b in inputs]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for j, (vp, tmp, im, itmp) in zip(jobs, plain):
+        for _ in range(steps):
+            fluid.fluid_timestep(vp, tmp, im, itmp, j.dt, j.visc, j.iters)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+
+    b = Batch()
+    batched = [clone(x) for x in inputs]
+    slots = [b.add(*bufs, j.dt, j.visc, j.iters) for j, bufs in zip(jobs, batched)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.sync()
+    t_batch = time.perf_counter() - t0
+    same = all(torch.equal(b.image(s), p[2].data) and torch.equal(bb[0].data, p[0].data)
+               for s, p, bb in zip(slots, plain, batched))
+    b.close()
+    cells = sum(j.fw * j.fh for j in jobs)
+    return {"scenes": n_scenes, "steps_per_scene": steps, "iters": 8, "cells": cells,
+            "jobs_per_s": n_scenes / t_batch, "plain_loop_jobs_per_s": n_scenes / t_plain,
+            "speedup_vs_plain_loop": t_plain / t_batch,
+            "cell_steps_per_s": cells * steps / t_batch, "results_identical": bool(same),
+            "entry_point": "pfs_batch_step (hipGraph replay)"}
+
+
+def served_leg(rate: float, duration: float, seed: int = 7) -> dict:
+    """The served workload: Poisson arrivals at `rate` jobs/s for `duration`
+    seconds (seed 7), dye pixels log-uniform in [8192, 2^20], field side =
+    round(sqrt(pixels)), steps uniform in [50, 500], K = 4, dt = 1/60, nu =
+    0.1; admission by a 200 GB HBM budget, continuous batching."""
+    from probabilistic_fluid_simulation_amd import scenes, serve
+
+    n = max(1, int(rate * duration))
+    jobs = serve.jobs_from_specs(scenes.served_specs(n, rate, seed=seed), iters=4)
+    srv = serve.Server(hbm_budget=200e9)
+    out = srv.run(jobs)
+    srv.close()
+    out["rate_jobs_per_s"] = rate
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +136,13 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=40,
                     help="steps of the CPU baseline sample (about 10 s on one core)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-scenes", type=int, default=256)
+    ap.add_argument("--batched-steps", type=int, default=300)
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--served-rates", default="64",
+                    help="comma-separated arrival rates (jobs/s) of the served leg")
+    ap.add_argument("--served-seconds", type=float, default=6.0)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, one device-to-device copy of this many MiB (a "
                          "kernel of known traffic for calibrating HBM byte counters)")
@@ -153,6 +226,12 @@ def main() -> int:
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scenes.plain_1m(steps=args.steps), args.cpu_steps)
+    del vp, tmp, im, itmp
+    if not args.no_batched:
+        out["batched"] = batched_leg(args.batched_scenes, args.batched_steps)
+    if not args.no_served:
+        out["served"] = [served_leg(float(r), args.served_seconds)
+                         for r in args.served_rates.split(",")]
     print(json.dumps(out), flush=True)
     if args.copy_probe_mib > 0:
         n = args.copy_probe_mib * (1 << 20) // 4

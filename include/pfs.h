@@ -145,6 +145,41 @@ size_t pfs_step_workspace_bytes(int vx, int vy);
 int pfs_profile_step(pfs_field *vp, pfs_field *tmp, pfs_field *image, pfs_field *itmp, float dt,
                      float viscosity, int iters, void *stream, float *ms, int n_ms);
 
+/* ---- batched scenes --------------------------------------------------------
+ * No counterpart in the reference, which steps one scene per call.  A batch
+ * holds many independent scenes (caller-owned device buffers, as for
+ * pfs_fluid_timestep) and advances all of them by one timestep each with one
+ * launch per kernel stage; the launch sequence is replayed from a hipGraph
+ * while membership is unchanged.  Every scene's results are bit-identical to
+ * stepping it alone with pfs_fluid_timestep.  Scenes need 4-channel,
+ * 16-byte aligned velocity fields and 1 <= iters <= 512; the dye may have any
+ * size and channel count. */
+typedef struct pfs_batch pfs_batch;
+
+/* stream NULL: the batch creates and owns a non-blocking stream. */
+int pfs_batch_create(pfs_batch **out, void *stream);
+/* Waits for the batch's work, frees its workspaces and tables. */
+int pfs_batch_destroy(pfs_batch *b);
+/* Adds a scene; the batch allocates its workspace (pfs_batch_workspace_bytes).
+ * *slot identifies it until pfs_batch_remove. */
+int pfs_batch_add(pfs_batch *b, const pfs_field *vp, const pfs_field *tmp, const pfs_field *image,
+                  const pfs_field *itmp, float dt, float viscosity, int iters, int *slot);
+/* Waits for the batch's queued work, then removes the scene. */
+int pfs_batch_remove(pfs_batch *b, int slot);
+/* Enqueues nsteps timesteps of every scene (simulate_fluid_step +
+ * advect_color_step each); returns without waiting. */
+int pfs_batch_step(pfs_batch *b, int nsteps);
+int pfs_batch_sync(pfs_batch *b);
+/* The scene's current dye buffer (the reference swaps image and itmp every
+ * step), the other one, and the number of steps taken. */
+int pfs_batch_scene(pfs_batch *b, int slot, pfs_field *image, pfs_field *itmp,
+                    long long *steps);
+int pfs_batch_count(pfs_batch *b);
+size_t pfs_batch_workspace_bytes(pfs_batch *b);
+/* 1 (default): replay steps from a hipGraph; 0: launch every kernel. */
+int pfs_batch_set_graphs(pfs_batch *b, int enable);
+void *pfs_batch_stream(pfs_batch *b);
+
 /* ---- debug: the integer "neighbour lists" ------------------------------
  * Back-trace cell keys of advect (src/fluid.cpp:55-57): keys[(j*w+i)*4 +
  * {0,1,2,3}] = {i0, j0, i1, j1} for every cell.  `keys` is a device int32

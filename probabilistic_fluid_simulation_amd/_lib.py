@@ -62,11 +62,23 @@ SIGNATURES = {
     "pfs_profile_step": [_P, _P, _P, _P, _F, _F, _I, _V, ctypes.POINTER(ctypes.c_float), _I],
     "pfs_debug_advect_keys": [_P, _F, _V, _V],
     "pfs_debug_advect_color_keys": [_P, _P, _F, _V, _V],
+    "pfs_batch_create": [ctypes.POINTER(ctypes.c_void_p), _V],
+    "pfs_batch_destroy": [_V],
+    "pfs_batch_add": [_V, _P, _P, _P, _P, _F, _F, _I, ctypes.POINTER(ctypes.c_int)],
+    "pfs_batch_remove": [_V, _I],
+    "pfs_batch_step": [_V, _I],
+    "pfs_batch_sync": [_V],
+    "pfs_batch_scene": [_V, _I, _P, _P, ctypes.POINTER(ctypes.c_longlong)],
+    "pfs_batch_count": [_V],
+    "pfs_batch_workspace_bytes": [_V],
+    "pfs_batch_set_graphs": [_V, _I],
+    "pfs_batch_stream": [_V],
     "pfs_version": [],
     "pfs_last_error": [],
 }
 _RESTYPES = {"pfs_version": ctypes.c_char_p, "pfs_last_error": ctypes.c_char_p,
-             "pfs_step_workspace_bytes": ctypes.c_size_t}
+             "pfs_step_workspace_bytes": ctypes.c_size_t,
+             "pfs_batch_workspace_bytes": ctypes.c_size_t, "pfs_batch_stream": ctypes.c_void_p}
 
 _lock = threading.Lock()
 _lib = None

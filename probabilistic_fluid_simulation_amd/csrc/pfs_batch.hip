@@ -1,0 +1,523 @@
+// Batched fluid step: many independent scenes advanced together.
+//
+// The reference advances one scene per call (src/main.cpp:219-240); small
+// scenes leave most of the GPU idle and are bound by launch latency.  A
+// pfs_batch holds a table of scenes (caller-owned buffers, batch-owned
+// workspaces) and advances every scene by one timestep — the reference's
+// simulate_fluid_step + advect_color_step — with one launch per kernel
+// stage for all scenes together:
+//
+//   k_diffuse_batch   every tile of every scene (scenes grouped by K)
+//   k_pressure_batch  every tile of every scene, dye fused where the dye has
+//                     the field's size
+//   k_color_batch     the dye of the remaining scenes
+//
+// Each workgroup finds its scene through a tile -> scene table and runs the
+// same per-tile body as the single-scene kernels (pfs_step_kernels.h), so
+// every scene's result is bit-identical to stepping it alone.  The launch
+// sequence of two consecutive steps (the dye buffers swap every step) is
+// captured once in a hipGraph and replayed while membership is unchanged;
+// adding or removing a scene rebuilds the tables and the graph.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "pfs.h"
+#include "pfs_internal.h"
+#include "pfs_step_kernels.h"
+
+namespace pfs {
+namespace {
+
+constexpr int kCBX = 64, kCBY = 4;  // dye blocks: 64 x 4 pixels
+
+template <int H, bool LAST>
+__global__ __launch_bounds__(Geo<H>::NT) void k_diffuse_batch(const DiffuseArgs *__restrict__ tab,
+                                                              const int *__restrict__ tile_scene,
+                                                              int ntiles) {
+    const int b = xcd_tile(blockIdx.x, ntiles);
+    const DiffuseArgs a = tab[tile_scene[b]];
+    diffuse_tile<H, LAST>(a, b - a.tile_base);
+}
+
+template <int H, bool LAST>
+__global__ __launch_bounds__(Geo<H>::NT) void k_pressure_batch(
+    const PressureArgs *__restrict__ tab, const int *__restrict__ tile_scene, int ntiles) {
+    const int b = xcd_tile(blockIdx.x, ntiles);
+    const PressureArgs a = tab[tile_scene[b]];
+    pressure_tile<H, LAST>(a, b - a.tile_base);
+}
+
+__global__ __launch_bounds__(kCBX *kCBY) void k_color_batch(const ColorArgs *__restrict__ tab,
+                                                            const int *__restrict__ block_scene) {
+    const ColorArgs a = tab[block_scene[blockIdx.x]];
+    const int blk = blockIdx.x - a.block_base;
+    const int i = (blk % a.blocks_x) * kCBX + threadIdx.x % kCBX;
+    const int j = (blk / a.blocks_x) * kCBY + threadIdx.x / kCBX;
+    if (i < a.iw && j < a.ih) advect_color_pixel(a, i, j);
+}
+
+using DiffuseBatchFn = void (*)(const DiffuseArgs *, const int *, int);
+using PressureBatchFn = void (*)(const PressureArgs *, const int *, int);
+
+#define PFS_H_CASES(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9)
+
+DiffuseBatchFn diffuse_batch_for(int H, bool last) {
+    switch (H) {
+#define PFS_CASE(h) \
+    case h:         \
+        return last ? k_diffuse_batch<h, true> : k_diffuse_batch<h, false>;
+        PFS_H_CASES(PFS_CASE)
+#undef PFS_CASE
+    }
+    return nullptr;
+}
+
+PressureBatchFn pressure_batch_for(int H, bool last) {
+    switch (H) {
+#define PFS_CASE(h) \
+    case h:         \
+        return last ? k_pressure_batch<h, true> : k_pressure_batch<h, false>;
+        PFS_H_CASES(PFS_CASE)
+#undef PFS_CASE
+    }
+    return nullptr;
+}
+
+constexpr int kMaxPlan = 64;  // chunks of one step: K up to 512
+
+}  // namespace
+}  // namespace pfs
+
+using namespace pfs;
+
+struct pfs_batch {
+    struct Scene {
+        bool live = false;
+        pfs_field vp{}, tmp{};
+        pfs_field dye[2]{};  // dye[(cur0 + steps) & 1] holds the current image
+        int cur0 = 0;
+        long long t0 = 0;  // batch step count when the scene joined
+        float dt = 0, visc = 0;
+        int iters = 0;
+        float *ws = nullptr;
+        size_t ws_bytes = 0;
+        bool dye_fused = false;
+        long long steps(long long t) const { return t - t0; }
+        int cur(long long t) const { return (int)((cur0 + t - t0) & 1); }
+    };
+    // Tables of one K group, all in device memory.
+    struct Group {
+        int iters = 0, nchunks = 0, ntiles = 0;
+        Chunk plan[kMaxPlan];
+        int *tile_scene = nullptr;
+        DiffuseArgs *dtab[kMaxPlan] = {};
+        PressureArgs *ptab[2][kMaxPlan] = {};
+    };
+    // Device memory or a graph that queued work may still use: released once
+    // an event recorded after that work has completed.
+    struct Retired {
+        hipEvent_t ev = nullptr;
+        void *mem = nullptr;
+        hipGraphExec_t graph = nullptr;
+    };
+
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::vector<Scene> scenes;
+    bool dirty = true;
+    bool use_graph = true;
+    std::vector<Group> groups;
+    ColorArgs *ctab[2] = {};
+    int *block_scene = nullptr;
+    int nblocks = 0;
+    void *dev_tables = nullptr;
+    hipGraphExec_t graph = nullptr;
+    long long t = 0;        // steps enqueued so far
+    long long t_build = 0;  // t when the tables were built
+    std::vector<Retired> retired;
+
+    // Frees retired items whose work has finished (all of them if `wait`).
+    int reap(bool wait) {
+        std::vector<Retired> keep;
+        for (auto &r : retired) {
+            hipError_t e = wait ? hipEventSynchronize(r.ev) : hipEventQuery(r.ev);
+            if (e == hipErrorNotReady) {
+                keep.push_back(r);
+                continue;
+            }
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                return set_error(PFS_ERR_HIP, std::string("batch event: ") + hipGetErrorString(e));
+            }
+            if (r.graph) (void)hipGraphExecDestroy(r.graph);
+            if (r.mem) (void)hipFree(r.mem);
+            (void)hipEventDestroy(r.ev);
+        }
+        retired.swap(keep);
+        return PFS_OK;
+    }
+
+    int retire(void *mem, hipGraphExec_t g) {
+        if (!mem && !g) return PFS_OK;
+        Retired r;
+        PFS_HIP_TRY(hipEventCreateWithFlags(&r.ev, hipEventDisableTiming));
+        PFS_HIP_TRY(hipEventRecord(r.ev, stream));
+        r.mem = mem;
+        r.graph = g;
+        retired.push_back(r);
+        return PFS_OK;
+    }
+
+    int sync() {
+        PFS_HIP_TRY(hipStreamSynchronize(stream));
+        return reap(true);
+    }
+
+    int drop_tables() {
+        PFS_TRY(retire(dev_tables, graph));
+        dev_tables = nullptr;
+        graph = nullptr;
+        groups.clear();
+        ctab[0] = ctab[1] = nullptr;
+        block_scene = nullptr;
+        nblocks = 0;
+        dirty = true;
+        return PFS_OK;
+    }
+
+    // Rebuilds every table for the live scenes.  Table parity q is the
+    // parity of (t - t_build): scene s then reads its image from
+    // dye[(cur0 + t_build - t0 + q) & 1].
+    int rebuild() {
+        PFS_TRY(reap(false));
+        PFS_TRY(drop_tables());
+        t_build = t;
+        std::map<int, std::vector<int>> by_k;
+        std::vector<int> unfused;
+        for (int i = 0; i < (int)scenes.size(); ++i) {
+            if (!scenes[i].live) continue;
+            by_k[scenes[i].iters].push_back(i);
+            if (!scenes[i].dye_fused) unfused.push_back(i);
+        }
+        std::vector<char> img;
+        auto reserve = [&](size_t bytes) {
+            size_t off = (img.size() + 255) & ~(size_t)255;
+            img.resize(off + bytes);
+            return off;
+        };
+        struct Off {
+            size_t tile_scene, dtab[kMaxPlan], ptab[2][kMaxPlan];
+        };
+        std::vector<Off> offs;
+        for (auto &kv : by_k) {
+            Group g;
+            g.iters = kv.first;
+            g.nchunks = chunk_plan(g.iters, g.plan, kMaxPlan);
+            if (!g.plan[g.nchunks - 1].last)
+                return set_error(PFS_ERR_UNSUPPORTED, "iters too large for the batch");
+            const std::vector<int> &ids = kv.second;
+            std::vector<int> tile_scene;
+            std::vector<DiffuseArgs> d(ids.size());
+            std::vector<PressureArgs> p(ids.size());
+            for (size_t k = 0; k < ids.size(); ++k) {
+                const Scene &s = scenes[ids[k]];
+                d[k] = diffuse_args(s.vp.data, s.tmp.data, s.vp.x, s.vp.y, s.dt, s.visc, s.iters,
+                                    s.ws);
+                d[k].tile_base = (int)tile_scene.size();
+                p[k] = pressure_args(s.vp.data, s.tmp.data, nullptr, nullptr, s.vp.x, s.vp.y, s.dt,
+                                     s.ws);
+                p[k].tile_base = d[k].tile_base;
+                tile_scene.insert(tile_scene.end(), d[k].ntiles, (int)k);
+            }
+            g.ntiles = (int)tile_scene.size();
+            Off o{};
+            o.tile_scene = reserve(tile_scene.size() * sizeof(int));
+            std::memcpy(img.data() + o.tile_scene, tile_scene.data(), tile_scene.size() * sizeof(int));
+            for (int c = 0; c < g.nchunks; ++c) {
+                o.dtab[c] = reserve(d.size() * sizeof(DiffuseArgs));
+                for (size_t k = 0; k < ids.size(); ++k) {
+                    DiffuseArgs a = d[k];
+                    diffuse_chunk(a, c, g.plan[c], scenes[ids[k]].ws);
+                    std::memcpy(img.data() + o.dtab[c] + k * sizeof(DiffuseArgs), &a, sizeof(a));
+                }
+                for (int q = 0; q < 2; ++q) {
+                    o.ptab[q][c] = reserve(d.size() * sizeof(PressureArgs));
+                    for (size_t k = 0; k < ids.size(); ++k) {
+                        const Scene &s = scenes[ids[k]];
+                        const int cur = (int)((s.cur(t_build) + q) & 1);
+                        PressureArgs a = p[k];
+                        pressure_chunk(a, c, g.plan[c], s.ws, s.dye_fused ? s.dye[cur].data : nullptr,
+                                       s.dye_fused ? s.dye[cur ^ 1].data : nullptr);
+                        std::memcpy(img.data() + o.ptab[q][c] + k * sizeof(PressureArgs), &a,
+                                    sizeof(a));
+                    }
+                }
+            }
+            groups.push_back(g);
+            offs.push_back(o);
+        }
+        size_t c_off[2] = {0, 0}, bs_off = 0;
+        if (!unfused.empty()) {
+            std::vector<int> block_scene_h;
+            std::vector<ColorArgs> c[2];
+            for (size_t k = 0; k < unfused.size(); ++k) {
+                const Scene &s = scenes[unfused[k]];
+                for (int q = 0; q < 2; ++q) {
+                    const int cur = (int)((s.cur(t_build) + q) & 1);
+                    const pfs_field &im = s.dye[cur];
+                    ColorArgs a = color_args(im.data, s.dye[cur ^ 1].data, s.vp.data, im.x, im.y,
+                                             im.z, s.vp.x, s.vp.y, s.vp.z, s.dt);
+                    a.blocks_x = (im.x + kCBX - 1) / kCBX;
+                    a.block_base = (int)block_scene_h.size();
+                    c[q].push_back(a);
+                }
+                const int nb = c[0].back().blocks_x * ((s.dye[0].y + kCBY - 1) / kCBY);
+                block_scene_h.insert(block_scene_h.end(), nb, (int)k);
+            }
+            nblocks = (int)block_scene_h.size();
+            bs_off = reserve(block_scene_h.size() * sizeof(int));
+            std::memcpy(img.data() + bs_off, block_scene_h.data(), block_scene_h.size() * sizeof(int));
+            for (int q = 0; q < 2; ++q) {
+                c_off[q] = reserve(c[q].size() * sizeof(ColorArgs));
+                std::memcpy(img.data() + c_off[q], c[q].data(), c[q].size() * sizeof(ColorArgs));
+            }
+        }
+        if (!img.empty()) {
+            if (hipMalloc(&dev_tables, img.size()) != hipSuccess) {
+                (void)hipGetLastError();
+                dev_tables = nullptr;
+                return set_error(PFS_ERR_NOMEM, "batch tables: device allocation failed");
+            }
+            // Synchronous: the new block is not yet used by any queued work, and
+            // the copy has completed (and read `img`) when the call returns.
+            PFS_HIP_TRY(hipMemcpy(dev_tables, img.data(), img.size(), hipMemcpyHostToDevice));
+        }
+        char *base = static_cast<char *>(dev_tables);
+        for (size_t gi = 0; gi < groups.size(); ++gi) {
+            Group &g = groups[gi];
+            g.tile_scene = reinterpret_cast<int *>(base + offs[gi].tile_scene);
+            for (int c = 0; c < g.nchunks; ++c) {
+                g.dtab[c] = reinterpret_cast<DiffuseArgs *>(base + offs[gi].dtab[c]);
+                for (int q = 0; q < 2; ++q)
+                    g.ptab[q][c] = reinterpret_cast<PressureArgs *>(base + offs[gi].ptab[q][c]);
+            }
+        }
+        if (nblocks) {
+            block_scene = reinterpret_cast<int *>(base + bs_off);
+            for (int q = 0; q < 2; ++q) ctab[q] = reinterpret_cast<ColorArgs *>(base + c_off[q]);
+        }
+        dirty = false;
+        return PFS_OK;
+    }
+
+    // One timestep of every scene, with the tables of parity q.
+    int enqueue_step(int q) {
+        for (const Group &g : groups) {
+            for (int c = 0; c < g.nchunks; ++c) {
+                hipLaunchKernelGGL(diffuse_batch_for(g.plan[c].H, g.plan[c].last), dim3(g.ntiles),
+                                   dim3(threads_for(g.plan[c].H)), 0, stream, g.dtab[c],
+                                   g.tile_scene, g.ntiles);
+                PFS_TRY(check_launch("batched diffuse"));
+            }
+            for (int c = 0; c < g.nchunks; ++c) {
+                hipLaunchKernelGGL(pressure_batch_for(g.plan[c].H, g.plan[c].last),
+                                   dim3(g.ntiles), dim3(threads_for(g.plan[c].H)), 0, stream,
+                                   g.ptab[q][c], g.tile_scene, g.ntiles);
+                PFS_TRY(check_launch("batched pressure"));
+            }
+        }
+        if (nblocks) {
+            hipLaunchKernelGGL(k_color_batch, dim3(nblocks), dim3(kCBX * kCBY), 0, stream,
+                               ctab[q], block_scene);
+            PFS_TRY(check_launch("batched dye"));
+        }
+        return PFS_OK;
+    }
+
+    int capture() {
+        hipGraph_t g = nullptr;
+        PFS_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_step(0);
+        if (rc == PFS_OK) rc = enqueue_step(1);
+        hipError_t e = hipStreamEndCapture(stream, &g);
+        if (rc != PFS_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess)
+            return set_error(PFS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            graph = nullptr;
+            return set_error(PFS_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+        return PFS_OK;
+    }
+
+    int step(int n) {
+        if (n < 0) return set_error(PFS_ERR_INVALID, "negative step count");
+        if (dirty) PFS_TRY(rebuild());
+        bool any = false;
+        for (auto &s : scenes) any = any || s.live;
+        if (!any) return PFS_OK;
+        for (int left = n; left > 0;) {
+            const int q = (int)((t - t_build) & 1);
+            if (use_graph && q == 0 && left >= 2) {
+                if (!graph) PFS_TRY(capture());
+                PFS_HIP_TRY(hipGraphLaunch(graph, stream));
+                t += 2;
+                left -= 2;
+            } else {
+                PFS_TRY(enqueue_step(q));
+                t += 1;
+                left -= 1;
+            }
+        }
+        return PFS_OK;
+    }
+};
+
+extern "C" {
+
+int pfs_batch_create(pfs_batch **out, void *stream) {
+    if (!out) return set_error(PFS_ERR_INVALID, "out is NULL");
+    pfs_batch *b = new pfs_batch();
+    if (stream) {
+        b->stream = (hipStream_t)stream;
+    } else {
+        if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete b;
+            return set_error(PFS_ERR_HIP, "hipStreamCreateWithFlags failed");
+        }
+        b->own_stream = true;
+    }
+    *out = b;
+    return PFS_OK;
+}
+
+int pfs_batch_destroy(pfs_batch *b) {
+    if (!b) return PFS_OK;
+    int rc = b->drop_tables();
+    if (rc == PFS_OK) rc = b->sync();
+    for (auto &s : b->scenes)
+        if (s.ws) (void)hipFree(s.ws);
+    if (b->own_stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return rc;
+}
+
+int pfs_batch_add(pfs_batch *b, const pfs_field *vp, const pfs_field *tmp, const pfs_field *image,
+                  const pfs_field *itmp, float dt, float viscosity, int iters, int *slot) {
+    if (!b || !slot) return set_error(PFS_ERR_INVALID, "batch or slot is NULL");
+    PFS_TRY(check_field(vp, "vp", 4));
+    PFS_TRY(check_field(tmp, "tmp", 4));
+    PFS_TRY(check_field(image, "image", 1));
+    PFS_TRY(check_field(itmp, "itmp", 1));
+    if (vp->x != tmp->x || vp->y != tmp->y || vp->z != tmp->z)
+        return set_error(PFS_ERR_INVALID, "tmp shape != vp shape");
+    if (image->x != itmp->x || image->y != itmp->y || image->z != itmp->z)
+        return set_error(PFS_ERR_INVALID, "itmp shape != image shape");
+    const float *ptrs[4] = {vp->data, tmp->data, image->data, itmp->data};
+    for (int i = 0; i < 4; ++i)
+        for (int j = i + 1; j < 4; ++j)
+            if (ptrs[i] == ptrs[j]) return set_error(PFS_ERR_INVALID, "scene buffers alias");
+    auto al = [](const void *p) { return (uintptr_t)p % 16 == 0; };
+    if (vp->z != 4 || iters < 1 || iters > kMaxChunk * kMaxPlan || !al(vp->data) || !al(tmp->data))
+        return set_error(PFS_ERR_UNSUPPORTED,
+                         "batched scenes need 4-channel, 16-byte aligned fields and 1 <= iters <= 512");
+    pfs_batch::Scene s;
+    s.live = true;
+    s.vp = *vp;
+    s.tmp = *tmp;
+    s.dye[0] = *image;
+    s.dye[1] = *itmp;
+    s.dt = dt;
+    s.visc = viscosity;
+    s.iters = iters;
+    s.dye_fused = image->x == vp->x && image->y == vp->y && image->z == 4 && al(image->data) &&
+                  al(itmp->data);
+    s.t0 = b->t;
+    s.ws_bytes = scene_ws_floats(vp->x, vp->y, iters) * sizeof(float);
+    if (hipMalloc(&s.ws, s.ws_bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(PFS_ERR_NOMEM, "scene workspace allocation failed");
+    }
+    // Reuse a free slot, else append.
+    int id = -1;
+    for (int i = 0; i < (int)b->scenes.size(); ++i)
+        if (!b->scenes[i].live) {
+            id = i;
+            break;
+        }
+    if (id < 0) {
+        id = (int)b->scenes.size();
+        b->scenes.push_back(s);
+    } else {
+        b->scenes[id] = s;
+    }
+    b->dirty = true;
+    *slot = id;
+    return PFS_OK;
+}
+
+int pfs_batch_remove(pfs_batch *b, int slot) {
+    if (!b || slot < 0 || slot >= (int)b->scenes.size() || !b->scenes[slot].live)
+        return set_error(PFS_ERR_INVALID, "no such scene");
+    // Queued steps may still use the workspace: it is released once they end.
+    PFS_TRY(b->retire(b->scenes[slot].ws, nullptr));
+    b->scenes[slot] = pfs_batch::Scene();
+    b->dirty = true;
+    return PFS_OK;
+}
+
+int pfs_batch_step(pfs_batch *b, int nsteps) {
+    if (!b) return set_error(PFS_ERR_INVALID, "batch is NULL");
+    return b->step(nsteps);
+}
+
+int pfs_batch_sync(pfs_batch *b) {
+    if (!b) return set_error(PFS_ERR_INVALID, "batch is NULL");
+    return b->sync();
+}
+
+int pfs_batch_scene(pfs_batch *b, int slot, pfs_field *image, pfs_field *itmp, long long *steps) {
+    if (!b || slot < 0 || slot >= (int)b->scenes.size() || !b->scenes[slot].live)
+        return set_error(PFS_ERR_INVALID, "no such scene");
+    const pfs_batch::Scene &s = b->scenes[slot];
+    const int cur = s.cur(b->t);
+    if (image) *image = s.dye[cur];
+    if (itmp) *itmp = s.dye[cur ^ 1];
+    if (steps) *steps = s.steps(b->t);
+    return PFS_OK;
+}
+
+int pfs_batch_count(pfs_batch *b) {
+    if (!b) return 0;
+    int n = 0;
+    for (auto &s : b->scenes) n += s.live ? 1 : 0;
+    return n;
+}
+
+size_t pfs_batch_workspace_bytes(pfs_batch *b) {
+    if (!b) return 0;
+    size_t n = 0;
+    for (auto &s : b->scenes) n += s.live ? s.ws_bytes : 0;
+    return n;
+}
+
+int pfs_batch_set_graphs(pfs_batch *b, int enable) {
+    if (!b) return set_error(PFS_ERR_INVALID, "batch is NULL");
+    b->use_graph = enable != 0;
+    return PFS_OK;
+}
+
+void *pfs_batch_stream(pfs_batch *b) { return b ? (void *)b->stream : nullptr; }
+
+}  // extern "C"

@@ -112,6 +112,51 @@ __device__ __forceinline__ float pressure_cell(float l, float r, float t, float 
     return (l + r + t + b + 1.0f * rhs) / 4.0f;
 }
 
+// Constants of advect_color (src/fluid.cpp:78-83,97-98), computed on the
+// host in float exactly as the reference computes them.
+struct ColorArgs {
+    const float *img;
+    float *out;
+    const float *vp;
+    int iw, ih, ic, vw, vh, vc;
+    float dt_viw, dt_vih, viw, vih, fiw, fih, inv_iw, inv_ih;
+    int pow2_iw, pow2_ih, vec4;
+    int blocks_x, block_base;  // 64x4-pixel blocks per row; first block in a batch
+};
+
+// One pixel of advect_color (src/fluid.cpp:86-125).
+__device__ __forceinline__ void advect_color_pixel(const ColorArgs &a, int i, int j) {
+    // Nearest velocity cell (src/fluid.cpp:89-90); clamped only against the
+    // float rounding of i*viw reaching vw, which the reference would read
+    // out of bounds.
+    const int vi = min((int)((float)i * a.viw), a.vw - 1);
+    const int vj = min((int)((float)j * a.vih), a.vh - 1);
+    const int vb = (vj * a.vw + vi) * a.vc;
+    const float xp = (float)i - div_n(a.dt_viw * a.vp[vb], a.fiw, a.inv_iw, a.pow2_iw);      // :97
+    const float yp = (float)j - div_n(a.dt_vih * a.vp[vb + 1], a.fih, a.inv_ih, a.pow2_ih);  // :98
+    const Trace t = trace(xp, yp, a.iw, a.ih, a.fiw, a.fih);
+    const int ic = a.ic;
+    const int base = (j * a.iw + i) * ic;
+    const int a00 = (t.j0 * a.iw + t.i0) * ic, a10 = (t.j0 * a.iw + t.i1) * ic;
+    const int a01 = (t.j1 * a.iw + t.i0) * ic, a11 = (t.j1 * a.iw + t.i1) * ic;
+    if (a.vec4) {
+        const float4 f00 = *reinterpret_cast<const float4 *>(a.img + a00);
+        const float4 f10 = *reinterpret_cast<const float4 *>(a.img + a10);
+        const float4 f01 = *reinterpret_cast<const float4 *>(a.img + a01);
+        const float4 f11 = *reinterpret_cast<const float4 *>(a.img + a11);
+        float4 r;
+        r.x = bilerp(t.sx, t.sy, f00.x, f10.x, f01.x, f11.x);
+        r.y = bilerp(t.sx, t.sy, f00.y, f10.y, f01.y, f11.y);
+        r.z = bilerp(t.sx, t.sy, f00.z, f10.z, f01.z, f11.z);
+        r.w = bilerp(t.sx, t.sy, f00.w, f10.w, f01.w, f11.w);
+        *reinterpret_cast<float4 *>(a.out + base) = r;
+    } else {
+        for (int k = 0; k < ic; ++k)
+            a.out[base + k] =
+                bilerp(t.sx, t.sy, a.img[a00 + k], a.img[a10 + k], a.img[a01 + k], a.img[a11 + k]);
+    }
+}
+
 // Visits every cell of the rectangle [x0, x0+rw) x [y0, y0+rh) once across
 // the NT threads of a block, row-major, with an incremental (x, y) update
 // instead of a division per cell.

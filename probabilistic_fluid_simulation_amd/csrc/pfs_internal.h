@@ -7,6 +7,7 @@
 #include <string>
 
 #include "pfs.h"
+#include "pfs_device.h"
 
 namespace pfs {
 
@@ -40,6 +41,9 @@ struct Pow2 {
 
 // Host-side constants of advect_color, computed in float exactly as
 // src/fluid.cpp:78-83,97-98 computes them.
+ColorArgs color_args(const float *img, float *out, const float *vp, int iw, int ih, int ic, int vw,
+                     int vh, int vc, float dt);
+
 struct ColorConsts {
     float fiw, fih, viw, vih, dt_viw, dt_vih;
     Pow2 piw, pih;
@@ -83,6 +87,7 @@ struct StepArgs {
 // are recorded around kernel group k (0 = diffuse, 1 = pressure).
 int fused_step(const StepArgs &a, float *ws, hipStream_t s, hipEvent_t *ev);
 size_t fused_workspace_bytes(int w, int h);
+int threads_for(int H);  // threads of the fused kernels for halo H
 
 // Validation helpers.
 int check_field(const pfs_field *f, const char *name, int min_channels);

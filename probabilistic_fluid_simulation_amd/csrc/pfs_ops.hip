@@ -34,41 +34,11 @@ __global__ __launch_bounds__(kBX *kBY) void k_advect(const float *__restrict__ f
         o[base + k] = bilerp(t.sx, t.sy, f[a00 + k], f[a10 + k], f[a01 + k], f[a11 + k]);
 }
 
-__global__ __launch_bounds__(kBX *kBY) void k_advect_color(
-    const float *__restrict__ img, float *__restrict__ out, const float *__restrict__ vp, int iw,
-    int ih, int ic, int vw, int vh, int vc, float dt_viw, float dt_vih, float viw, float vih,
-    float fiw, float fih, float inv_iw, float inv_ih, int pow2_iw, int pow2_ih, int vec4) {
+__global__ __launch_bounds__(kBX *kBY) void k_advect_color(ColorArgs a) {
     const int i = blockIdx.x * kBX + threadIdx.x;
     const int j = blockIdx.y * kBY + threadIdx.y;
-    if (i >= iw || j >= ih) return;
-    // Nearest velocity cell (src/fluid.cpp:89-90); clamped only against the
-    // float rounding of i*viw reaching vw, which the reference would read
-    // out of bounds.
-    const int vi = min((int)((float)i * viw), vw - 1);
-    const int vj = min((int)((float)j * vih), vh - 1);
-    const int vb = (vj * vw + vi) * vc;
-    const float xp = (float)i - div_n(dt_viw * vp[vb], fiw, inv_iw, pow2_iw);      // :97
-    const float yp = (float)j - div_n(dt_vih * vp[vb + 1], fih, inv_ih, pow2_ih);  // :98
-    const Trace t = trace(xp, yp, iw, ih, fiw, fih);
-    const int base = (j * iw + i) * ic;
-    const int a00 = (t.j0 * iw + t.i0) * ic, a10 = (t.j0 * iw + t.i1) * ic;
-    const int a01 = (t.j1 * iw + t.i0) * ic, a11 = (t.j1 * iw + t.i1) * ic;
-    if (vec4) {
-        const float4 f00 = *reinterpret_cast<const float4 *>(img + a00);
-        const float4 f10 = *reinterpret_cast<const float4 *>(img + a10);
-        const float4 f01 = *reinterpret_cast<const float4 *>(img + a01);
-        const float4 f11 = *reinterpret_cast<const float4 *>(img + a11);
-        float4 r;
-        r.x = bilerp(t.sx, t.sy, f00.x, f10.x, f01.x, f11.x);
-        r.y = bilerp(t.sx, t.sy, f00.y, f10.y, f01.y, f11.y);
-        r.z = bilerp(t.sx, t.sy, f00.z, f10.z, f01.z, f11.z);
-        r.w = bilerp(t.sx, t.sy, f00.w, f10.w, f01.w, f11.w);
-        *reinterpret_cast<float4 *>(out + base) = r;
-    } else {
-        for (int k = 0; k < ic; ++k)
-            out[base + k] =
-                bilerp(t.sx, t.sy, img[a00 + k], img[a10 + k], img[a01 + k], img[a11 + k]);
-    }
+    if (i >= a.iw || j >= a.ih) return;
+    advect_color_pixel(a, i, j);
 }
 
 // One Jacobi smoothing sweep of channels 0,1 (src/fluid.cpp:154-185).
@@ -196,14 +166,39 @@ int launch_advect(const float *f, float *o, int w, int h, int c, float dt, hipSt
     return check_launch("advect");
 }
 
+ColorArgs color_args(const float *img, float *out, const float *vp, int iw, int ih, int ic, int vw,
+                     int vh, int vc, float dt) {
+    const ColorConsts k(iw, ih, vw, vh, dt);
+    ColorArgs a{};
+    a.img = img;
+    a.out = out;
+    a.vp = vp;
+    a.iw = iw;
+    a.ih = ih;
+    a.ic = ic;
+    a.vw = vw;
+    a.vh = vh;
+    a.vc = vc;
+    a.dt_viw = k.dt_viw;
+    a.dt_vih = k.dt_vih;
+    a.viw = k.viw;
+    a.vih = k.vih;
+    a.fiw = k.fiw;
+    a.fih = k.fih;
+    a.inv_iw = k.piw.inv;
+    a.inv_ih = k.pih.inv;
+    a.pow2_iw = k.piw.is;
+    a.pow2_ih = k.pih.is;
+    // float4 loads/stores need 4 channels and 16-byte aligned buffers.
+    a.vec4 = ic == 4 && (uintptr_t)img % 16 == 0 && (uintptr_t)out % 16 == 0;
+    a.blocks_x = (iw + kBX - 1) / kBX;
+    return a;
+}
+
 int launch_advect_color(const float *img, float *out, const float *vp, int iw, int ih, int ic,
                         int vw, int vh, int vc, float dt, hipStream_t s) {
-    const ColorConsts k(iw, ih, vw, vh, dt);
-    // float4 loads/stores need 4 channels and 16-byte aligned buffers.
-    const int vec4 = ic == 4 && (uintptr_t)img % 16 == 0 && (uintptr_t)out % 16 == 0;
-    hipLaunchKernelGGL(k_advect_color, grid_for(iw, ih), kBlock, 0, s, img, out, vp, iw, ih, ic,
-                       vw, vh, vc, k.dt_viw, k.dt_vih, k.viw, k.vih, k.fiw, k.fih, k.piw.inv,
-                       k.pih.inv, k.piw.is, k.pih.is, vec4);
+    const ColorArgs a = color_args(img, out, vp, iw, ih, ic, vw, vh, vc, dt);
+    hipLaunchKernelGGL(k_advect_color, grid_for(iw, ih), kBlock, 0, s, a);
     return check_launch("advect_color");
 }
 

@@ -33,558 +33,94 @@
 
 #include <cmath>
 
-#include "pfs_device.h"
 #include "pfs_internal.h"
+#include "pfs_step_kernels.h"
 
 namespace pfs {
 
 namespace {
 
-constexpr int kTile = 64;    // tile side
-constexpr int kSegs = 12;    // row segments per staged column
-constexpr int kMaxChunk = 8; // sweeps per launch
-
-template <int H_>
-struct Geo {
-    static constexpr int H = H_;
-    static constexpr int TX = kTile, TY = kTile;
-    static constexpr int RX = TX + 2 * H, RY = TY + 2 * H;
-    static constexpr int S = (RY + kSegs - 1) / kSegs;  // rows per thread
-    static constexpr int RYP = kSegs * S;               // rows of an LDS plane (>= RY)
-    static constexpr int NTH = RX * kSegs;              // threads owning cells
-    static constexpr int NT = ((NTH + 63) / 64) * 64;   // whole wavefronts
-};
-
-// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
-// XCD b % 8).  Remap so that each XCD owns a contiguous band of tiles and the
-// halos its tiles share stay in that XCD's L2.
-__device__ __forceinline__ int xcd_tile(int b, int n) {
-    if ((n & 7) != 0) return b;
-    return (b & 7) * (n >> 3) + (b >> 3);
-}
-
-__device__ __forceinline__ int wrap_index(int g, int n) {
-    g %= n;
-    return g < 0 ? g + n : g;
-}
-
-struct DiffuseArgs {
-    const float *P;      // vp, [h][w][4]
-    float *Q;            // tmp, [h][w][4]
-    const float2 *dsrc;  // iterate plane read by a non-first chunk
-    float2 *ddst;        // iterate plane written by a non-last chunk
-    float *pw;           // pressure-start plane (last chunk)
-    float *bpl;          // divergence plane (last chunk)
-    int w, h, tiles_x, ntiles;
-    float dt, fw, fh, inv_w, inv_h;
-    int pow2_w, pow2_h;
-    float alpha, beta, rcp_beta, gamma;
-    int fast_div;  // 0: IEEE division; 1: 5-op exact division; 2: 3-op exact division
-    int first, k_even;
-};
-
-struct PressureArgs {
-    float *P;
-    float *Q;
-    const float *psrc;  // iterate plane read by a non-first chunk
-    float *pdst;        // iterate plane written by a non-last chunk
-    const float *pw;
-    const float *bpl;
-    const float *image;  // dye, [h][w][4] (fused dye step only)
-    float *itmp;
-    int w, h, tiles_x, ntiles;
-    float dt, fw, fh, inv_w, inv_h;
-    int pow2_w, pow2_h;
-    float dt_viw, dt_vih;  // dt / (field/image width ratio), as src/fluid.cpp:97-98
-    int first;
-};
-
-// Thread (x, seg) of a tile: column x of the staged region, rows
-// [r0, r0 + S), and the global cells those map to (periodic wrap).
-template <class G>
-struct Strip {
-    bool active;
-    int x, r0, xl, xr;
-    int x0, y0;  // global coordinates of region cell (0, 0), unwrapped
-    int gi;      // global column of x
-    int gj[G::S];
-    __device__ Strip(int tid, int tile, int tiles_x, int w, int h) {
-        active = tid < G::NTH;
-        x = active ? tid % G::RX : 0;
-        r0 = active ? (tid / G::RX) * G::S : 0;
-        xl = x > 0 ? x - 1 : x;
-        xr = x < G::RX - 1 ? x + 1 : x;
-        x0 = (tile % tiles_x) * G::TX - G::H;
-        y0 = (tile / tiles_x) * G::TY - G::H;
-        gi = wrap_index(x0 + x, w);
-        int g = wrap_index(y0 + r0, h);
-#pragma unroll
-        for (int k = 0; k < G::S; ++k) {
-            gj[k] = g;
-            g = (g + 1 == h) ? 0 : g + 1;
-        }
-    }
-    __device__ bool row_ok(int k) const { return active && r0 + k < G::RY; }
-    // Row r0+k of column x is a tile cell inside the field (not halo or
-    // past the field's edge); then its global cell is (x0+x, y0+r0+k).
-    __device__ bool mine(int k, int w, int h) const {
-        const int r = r0 + k;
-        return active && r >= G::H && r < G::H + G::TY && x >= G::H && x < G::H + G::TX &&
-               x0 + x < w && y0 + r < h;
-    }
-    __device__ int cell(int k, int w) const { return (y0 + r0 + k) * w + x0 + x; }
-};
-
-// Region coordinate of global index g (0 <= g < n) on an axis whose staged
-// region starts at global coordinate o (periodic), or `lim` when g + 1 is
-// not staged right after it.
-__device__ __forceinline__ int staged(int g, int o, int n, int lim) {
-    int r = g - o;  // in (-n, n + H) for o in [-H, n)
-    if (r < 0) r += n;
-    if (r >= n) r -= n;
-    return r < lim - 1 ? r : lim;
+template <int H, bool LAST>
+__global__ __launch_bounds__(Geo<H>::NT) void k_diffuse(DiffuseArgs a) {
+    diffuse_tile<H, LAST>(a, xcd_tile(blockIdx.x, a.ntiles));
 }
 
 template <int H, bool LAST>
-__global__ __launch_bounds__(Geo<H>::NT) void k_diffuse(DiffuseArgs a) {
-    using G = Geo<H>;
-    constexpr int S = G::S, RX = G::RX, RY = G::RY, PL = G::RYP * RX;
-    constexpr int NS = H - (LAST ? 1 : 0);  // sweeps in this launch
-    __shared__ float2 plane[2 * PL];
-
-    const Strip<G> st(threadIdx.x, xcd_tile(blockIdx.x, a.ntiles), a.tiles_x, a.w, a.h);
-    const float2 *__restrict__ P2 = reinterpret_cast<const float2 *>(a.P);  // (u,v) at 2*cell
-    float4 *__restrict__ Q4 = reinterpret_cast<float4 *>(a.Q);
-
-    // Every global load of the launch is issued here: the staged region, and
-    // (last chunk) the tile cells' tmp and pressure start, whose latency
-    // then hides behind the advection and the sweeps.
-    float2 c[S];
-#pragma unroll
-    for (int k = 0; k < S; ++k) {
-        const int g = st.row_ok(k) ? st.gj[k] * a.w + st.gi : 0;
-        c[k] = a.first ? P2[2 * g] : a.dsrc[g];
-    }
-    float4 q[S];
-    float warm[S];
-    if (LAST) {
-#pragma unroll
-        for (int k = 0; k < S; ++k) {
-            const bool m = st.mine(k, a.w, a.h);
-            q[k] = m ? Q4[st.cell(k, a.w)] : make_float4(0.f, 0.f, 0.f, 0.f);
-            warm[k] = (m && !a.k_even) ? a.P[4 * st.cell(k, a.w) + 2] : q[k].z;
-        }
-    }
-
-    float u[S], v[S];
-    if (a.first) {
-        // Advect every strip cell (src/fluid.cpp:36-67) with its corners
-        // from the staged velocities in LDS plane 1 — or from memory when a
-        // back-trace leaves the staged region.
-        float2 *V = plane + PL;
-#pragma unroll
-        for (int k = 0; k < S; ++k)
-            if (st.active) V[(st.r0 + k) * RX + st.x] = c[k];
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < S; ++k) {
-            const float xp = (float)st.gi - div_n(a.dt * c[k].x, a.fw, a.inv_w, a.pow2_w);
-            const float yp = (float)st.gj[k] - div_n(a.dt * c[k].y, a.fh, a.inv_h, a.pow2_h);
-            const Trace t = trace(xp, yp, a.w, a.h, a.fw, a.fh);
-            const int rx = staged(t.i0, st.x0, a.w, RX), ry = staged(t.j0, st.y0, a.h, RY);
-            float2 f00, f10, f01, f11;
-            if (rx < RX && ry < RY) {
-                const int o = ry * RX + rx;
-                f00 = V[o];
-                f10 = V[o + 1];
-                f01 = V[o + RX];
-                f11 = V[o + RX + 1];
-            } else {
-                f00 = P2[2 * (t.j0 * a.w + t.i0)];
-                f10 = P2[2 * (t.j0 * a.w + t.i1)];
-                f01 = P2[2 * (t.j1 * a.w + t.i0)];
-                f11 = P2[2 * (t.j1 * a.w + t.i1)];
-            }
-            u[k] = bilerp(t.sx, t.sy, f00.x, f10.x, f01.x, f11.x);
-            v[k] = bilerp(t.sx, t.sy, f00.y, f10.y, f01.y, f11.y);
-        }
-    } else {
-#pragma unroll
-        for (int k = 0; k < S; ++k) {
-            u[k] = c[k].x;
-            v[k] = c[k].y;
-        }
-    }
-
-    // Jacobi smoothing sweeps (src/fluid.cpp:154-185):
-    // out = (a*L + a*R + a*T + a*B + 1*C) / beta, left to right.  Sweep s
-    // publishes alpha-scaled cells into plane (s+1)&1: sweep 1 uses plane 0
-    // while plane 1 may still be read by the advection above.  The division
-    // takes the 5-FMA exactly rounded path; if any numerator of the strip
-    // is outside that path's range (zero, tiny, huge, Inf) the strip is redone
-    // with IEEE division.  Rows past RY are padding: computed, never used.
-    float pu[S], pv[S];  // iterate before the last sweep (E for odd K)
-    const bool fast = a.fast_div != 0;
-    for (int s = 1; s <= NS; ++s) {
-        float2 *B = plane + ((s + 1) & 1) * PL;
-        float au[S], av[S];
-#pragma unroll
-        for (int k = 0; k < S; ++k) {
-            au[k] = a.alpha * u[k];
-            av[k] = a.alpha * v[k];
-            if (st.active) B[(st.r0 + k) * RX + st.x] = make_float2(au[k], av[k]);
-        }
-        __syncthreads();
-        if (!st.active) continue;
-        const float2 top = B[(st.r0 > 0 ? st.r0 - 1 : 0) * RX + st.x];
-        const float2 bot = B[(st.r0 + S < RY ? st.r0 + S : RY - 1) * RX + st.x];
-        const float2 *Bl = B + st.r0 * RX + st.xl, *Br = B + st.r0 * RX + st.xr;
-        float nu[S], nv[S];
-        float lo = 0x1p127f, hi = 0.0f;  // magnitude range of the numerators
-        const bool short_div = a.fast_div == 2;
-#pragma unroll
-        for (int k = 0; k < S; ++k) {
-            if (LAST && s == NS) {
-                pu[k] = u[k];
-                pv[k] = v[k];
-            }
-            const float2 L = Bl[k * RX], R = Br[k * RX];
-            const float tu = k == 0 ? top.x : au[k - 1], tv = k == 0 ? top.y : av[k - 1];
-            const float bu = k == S - 1 ? bot.x : au[k + 1];
-            const float bv = k == S - 1 ? bot.y : av[k + 1];
-            nu[k] = L.x + R.x + tu + bu + 1.0f * u[k];
-            nv[k] = L.y + R.y + tv + bv + 1.0f * v[k];
-            lo = fminf(lo, fminf(fabsf(nu[k]), fabsf(nv[k])));
-            hi = fmaxf(hi, fmaxf(fabsf(nu[k]), fabsf(nv[k])));
-        }
-        if (short_div) {
-#pragma unroll
-            for (int k = 0; k < S; ++k) {
-                u[k] = div_fast3(nu[k], a.beta, a.rcp_beta);
-                v[k] = div_fast3(nv[k], a.beta, a.rcp_beta);
-            }
-        } else {
-#pragma unroll
-            for (int k = 0; k < S; ++k) {
-                u[k] = div_fast(nu[k], a.beta, a.rcp_beta);
-                v[k] = div_fast(nv[k], a.beta, a.rcp_beta);
-            }
-        }
-        if (!fast || !(lo >= 0x1p-96f) || !(hi < 0x1p96f)) {
-#pragma unroll
-            for (int k = 0; k < S; ++k) {
-                u[k] = div_fixed(nu[k], a.beta, a.rcp_beta, fast);
-                v[k] = div_fixed(nv[k], a.beta, a.rcp_beta, fast);
-            }
-        }
-    }
-
-    if (!LAST) {
-#pragma unroll
-        for (int k = 0; k < S; ++k)
-            if (st.mine(k, a.w, a.h)) a.ddst[st.cell(k, a.w)] = make_float2(u[k], v[k]);
-        return;
-    }
-
-    // Divergence of D^K on the tile (src/fluid.cpp:221-237).  The plane
-    // written here was last read in sweep NS-1, which every thread has left.
-    float2 *B = plane + (NS & 1) * PL;
-#pragma unroll
-    for (int k = 0; k < S; ++k)
-        if (st.active) B[(st.r0 + k) * RX + st.x] = make_float2(u[k], v[k]);
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < S; ++k) {
-        if (!st.mine(k, a.w, a.h)) continue;
-        const int o = (st.r0 + k) * RX + st.x;
-        const float uR = B[o + 1].x - B[o - 1].x;
-        const float vT = B[o + RX].y - B[o - RX].y;
-        const float b = a.gamma * (uR + vT);
-        const int cell = st.cell(k, a.w);
-        a.pw[cell] = warm[k];
-        a.bpl[cell] = b;
-        Q4[cell] = make_float4(a.k_even ? u[k] : pu[k], a.k_even ? v[k] : pv[k], q[k].z, b);
-    }
-}
-
-template <int H, bool LAST, bool DYE>
 __global__ __launch_bounds__(Geo<H>::NT) void k_pressure(PressureArgs a) {
-    using G = Geo<H>;
-    constexpr int S = G::S, RX = G::RX, RY = G::RY, PL = G::RYP * RX;
-    constexpr int NS = H - (LAST ? 1 : 0);
-    __shared__ float plane[2 * PL];
-
-    const Strip<G> st(threadIdx.x, xcd_tile(blockIdx.x, a.ntiles), a.tiles_x, a.w, a.h);
-    float4 *__restrict__ P4 = reinterpret_cast<float4 *>(a.P);
-    float4 *__restrict__ Q4 = reinterpret_cast<float4 *>(a.Q);
-
-    // All global loads up front (see k_diffuse).
-    float p[S], rhs[S];
-    {
-        const float *__restrict__ src = a.first ? a.pw : a.psrc;
-#pragma unroll
-        for (int k = 0; k < S; ++k) {
-            const int g = st.row_ok(k) ? st.gj[k] * a.w + st.gi : 0;
-            p[k] = src[g];
-            rhs[k] = a.bpl[g];
-        }
-    }
-    float4 q[S];  // (E.u, E.v, start, b) of the tile cells
-    if (LAST) {
-#pragma unroll
-        for (int k = 0; k < S; ++k)
-            q[k] = st.mine(k, a.w, a.h) ? Q4[st.cell(k, a.w)] : make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-
-    // Pressure Jacobi sweeps (src/fluid.cpp:240-257):
-    // p' = (pL + pR + pT + pB + 1*b) / 4, left to right.
-    float pp[S];  // p^(K-1)
-    for (int s = 1; s <= NS; ++s) {
-        float *B = plane + (s & 1) * PL;
-#pragma unroll
-        for (int k = 0; k < S; ++k)
-            if (st.active) B[(st.r0 + k) * RX + st.x] = p[k];
-        __syncthreads();
-        if (!st.active) continue;
-        const float top = B[(st.r0 > 0 ? st.r0 - 1 : 0) * RX + st.x];
-        const float bot = B[(st.r0 + S < RY ? st.r0 + S : RY - 1) * RX + st.x];
-        const float *Bl = B + st.r0 * RX + st.xl, *Br = B + st.r0 * RX + st.xr;
-        float prev = top;
-#pragma unroll
-        for (int k = 0; k < S; ++k) {
-            const float cur = p[k];
-            if (LAST && s == NS) pp[k] = cur;
-            const float below = k == S - 1 ? bot : p[k + 1];
-            p[k] = pressure_cell(Bl[k * RX], Br[k * RX], prev, below, rhs[k]);
-            prev = cur;
-        }
-    }
-
-    if (!LAST) {
-#pragma unroll
-        for (int k = 0; k < S; ++k)
-            if (st.mine(k, a.w, a.h)) a.pdst[st.cell(k, a.w)] = p[k];
-        return;
-    }
-
-    // Gradient subtraction on the tile (src/fluid.cpp:273-295).
-    float *B = plane + ((NS + 1) & 1) * PL;
-#pragma unroll
-    for (int k = 0; k < S; ++k)
-        if (st.active) B[(st.r0 + k) * RX + st.x] = p[k];
-    __syncthreads();
-    float un[S], vn[S];
-#pragma unroll
-    for (int k = 0; k < S; ++k) {
-        un[k] = vn[k] = 0.0f;
-        if (!st.mine(k, a.w, a.h)) continue;
-        const int o = (st.r0 + k) * RX + st.x;
-        const float gradX = (B[o + 1] - B[o - 1]) * a.dt / 2.0f;
-        const float gradY = (B[o + RX] - B[o - RX]) * a.dt / 2.0f;
-        const int cell = st.cell(k, a.w);
-        un[k] = q[k].x - gradX;
-        vn[k] = q[k].y - gradY;
-        P4[cell] = make_float4(un[k], vn[k], pp[k], rhs[k]);
-        Q4[cell] = make_float4(q[k].x, q[k].y, p[k], rhs[k]);
-    }
-    if (!DYE) return;
-
-    // Dye advection of the tile's pixels (src/fluid.cpp:89-124 with the
-    // image the size of the field, so pixel (i, j) samples cell (i, j)),
-    // pixels spread over all threads so that their gathers are in flight
-    // together.  The new velocities are handed over through LDS: both
-    // planes (>= TX*TY float2) are free once every thread is past the
-    // gradient.
-    static_assert(2 * PL >= 2 * G::TX * G::TY, "velocity hand-over does not fit");
-    __syncthreads();
-    float2 *Vn = reinterpret_cast<float2 *>(plane);
-#pragma unroll
-    for (int k = 0; k < S; ++k)
-        if (st.mine(k, a.w, a.h))
-            Vn[(st.r0 + k - H) * G::TX + (st.x - H)] = make_float2(un[k], vn[k]);
-    __syncthreads();
-    const float4 *__restrict__ I4 = reinterpret_cast<const float4 *>(a.image);
-    float4 *__restrict__ O4 = reinterpret_cast<float4 *>(a.itmp);
-    const int tx0 = st.x0 + H, ty0 = st.y0 + H;
-    constexpr int PER = (G::TX * G::TY + G::NT - 1) / G::NT;
-#pragma unroll
-    for (int m = 0; m < PER; ++m) {
-        const int idx = threadIdx.x + m * G::NT;
-        if (idx >= G::TX * G::TY) break;
-        const int gx = tx0 + idx % G::TX, gy = ty0 + idx / G::TX;
-        if (gx >= a.w || gy >= a.h) continue;
-        const float2 vel = Vn[idx];
-        const float xp = (float)gx - div_n(a.dt_viw * vel.x, a.fw, a.inv_w, a.pow2_w);
-        const float yp = (float)gy - div_n(a.dt_vih * vel.y, a.fh, a.inv_h, a.pow2_h);
-        const Trace t = trace(xp, yp, a.w, a.h, a.fw, a.fh);
-        const float4 f00 = I4[t.j0 * a.w + t.i0], f10 = I4[t.j0 * a.w + t.i1];
-        const float4 f01 = I4[t.j1 * a.w + t.i0], f11 = I4[t.j1 * a.w + t.i1];
-        O4[gy * a.w + gx] = make_float4(bilerp(t.sx, t.sy, f00.x, f10.x, f01.x, f11.x),
-                                        bilerp(t.sx, t.sy, f00.y, f10.y, f01.y, f11.y),
-                                        bilerp(t.sx, t.sy, f00.z, f10.z, f01.z, f11.z),
-                                        bilerp(t.sx, t.sy, f00.w, f10.w, f01.w, f11.w));
-    }
+    pressure_tile<H, LAST>(a, xcd_tile(blockIdx.x, a.ntiles));
 }
 
 // ---- host dispatch over the compile-time halo ------------------------------
 using DiffuseFn = void (*)(DiffuseArgs);
 using PressureFn = void (*)(PressureArgs);
 
-template <int H>
-DiffuseFn diffuse_fn(bool last) {
-    return last ? k_diffuse<H, true> : k_diffuse<H, false>;
-}
-template <int H>
-PressureFn pressure_fn(bool last, bool dye) {
-    if (!last) return k_pressure<H, false, false>;
-    return dye ? k_pressure<H, true, true> : k_pressure<H, true, false>;
+#define PFS_H_CASES(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9)
+
+DiffuseFn diffuse_for(int H, bool last) {
+    switch (H) {
+#define PFS_CASE(h) \
+    case h:         \
+        return last ? k_diffuse<h, true> : k_diffuse<h, false>;
+        PFS_H_CASES(PFS_CASE)
+#undef PFS_CASE
+    }
+    return nullptr;
 }
 
-// Launch geometry by halo H = sweeps + (last ? 1 : 0), 1 <= H <= kMaxChunk+1.
+PressureFn pressure_for(int H, bool last) {
+    switch (H) {
+#define PFS_CASE(h) \
+    case h:         \
+        return last ? k_pressure<h, true> : k_pressure<h, false>;
+        PFS_H_CASES(PFS_CASE)
+#undef PFS_CASE
+    }
+    return nullptr;
+}
+
+}  // namespace
+
 int threads_for(int H) {
     switch (H) {
 #define PFS_CASE(h) \
     case h:         \
         return Geo<h>::NT;
-        PFS_CASE(2) PFS_CASE(3) PFS_CASE(4) PFS_CASE(5)
-        PFS_CASE(6) PFS_CASE(7) PFS_CASE(8) PFS_CASE(9)
+        PFS_H_CASES(PFS_CASE)
 #undef PFS_CASE
     }
     return 0;
 }
 
-DiffuseFn diffuse_for(int H, bool last) {
-    switch (H) {
-        case 2: return diffuse_fn<2>(last);
-        case 3: return diffuse_fn<3>(last);
-        case 4: return diffuse_fn<4>(last);
-        case 5: return diffuse_fn<5>(last);
-        case 6: return diffuse_fn<6>(last);
-        case 7: return diffuse_fn<7>(last);
-        case 8: return diffuse_fn<8>(last);
-        case 9: return diffuse_fn<9>(last);
-    }
-    return nullptr;
+size_t fused_workspace_bytes(int w, int h) {
+    return scene_ws_floats(w, h, kMaxChunk + 1) * sizeof(float);
 }
-
-PressureFn pressure_for(int H, bool last, bool dye) {
-    switch (H) {
-        case 2: return pressure_fn<2>(last, dye);
-        case 3: return pressure_fn<3>(last, dye);
-        case 4: return pressure_fn<4>(last, dye);
-        case 5: return pressure_fn<5>(last, dye);
-        case 6: return pressure_fn<6>(last, dye);
-        case 7: return pressure_fn<7>(last, dye);
-        case 8: return pressure_fn<8>(last, dye);
-        case 9: return pressure_fn<9>(last, dye);
-    }
-    return nullptr;
-}
-
-// Workspace layout (float offsets, N = w*h): pw [0, N), b [N, 2N),
-// iterate planes d0 [2N, 4N), d1 [4N, 6N) (float2), p0 [6N, 7N), p1 [7N, 8N).
-size_t ws_floats(int w, int h) { return 8 * (size_t)w * (size_t)h; }
-
-}  // namespace
-
-size_t fused_workspace_bytes(int w, int h) { return ws_floats(w, h) * sizeof(float); }
 
 int fused_step(const StepArgs &s, float *ws, hipStream_t stream, hipEvent_t *ev) {
-    const int w = s.w, h = s.h, K = s.iters;
-    if (K < 1) return set_error(PFS_ERR_INVALID, "fused step needs iters >= 1");
-    const size_t N = (size_t)w * h;
-    const int tiles_x = (w + kTile - 1) / kTile, tiles_y = (h + kTile - 1) / kTile;
-    const int ntiles = tiles_x * tiles_y;
-    float *pw = ws, *bpl = ws + N;
-    float2 *dpl[2] = {reinterpret_cast<float2 *>(ws + 2 * N),
-                      reinterpret_cast<float2 *>(ws + 4 * N)};
-    float *ppl[2] = {ws + 6 * N, ws + 7 * N};
-    const Pow2 p2w(w), p2h(h);
-    const float alpha = diffuse_alpha(s.visc, s.dt);
-    const float beta = diffuse_beta(alpha);
-
-    DiffuseArgs d{};
-    d.P = s.P;
-    d.Q = s.Q;
-    d.pw = pw;
-    d.bpl = bpl;
-    d.w = w;
-    d.h = h;
-    d.tiles_x = tiles_x;
-    d.ntiles = ntiles;
-    d.dt = s.dt;
-    d.fw = (float)w;
-    d.fh = (float)h;
-    d.inv_w = p2w.inv;
-    d.inv_h = p2h.inv;
-    d.pow2_w = p2w.is;
-    d.pow2_h = p2h.is;
-    d.alpha = alpha;
-    d.beta = beta;
-    d.rcp_beta = 1.0f / beta;
-    d.fast_div = 0;
-    if (std::isfinite(beta) && beta >= 0x1p-20f && beta <= 0x1p20f) {
-        // y*beta is exact in double (24 x 24 bits).
-        const double e = (double)d.rcp_beta * (double)beta - 1.0;
-        d.fast_div = std::fabs(e) <= 0x1p-26 ? 2 : 1;
-    }
-    d.gamma = pressure_gamma(s.dt);
-    d.k_even = (K % 2) == 0;
+    if (s.iters < 1) return set_error(PFS_ERR_INVALID, "fused step needs iters >= 1");
+    Chunk plan[64];
+    const int nchunks = chunk_plan(s.iters, plan, 64);
+    if (plan[nchunks - 1].last == false)
+        return set_error(PFS_ERR_UNSUPPORTED, "iters too large for one step plan");
+    DiffuseArgs d = diffuse_args(s.P, s.Q, s.w, s.h, s.dt, s.visc, s.iters, ws);
+    PressureArgs p = pressure_args(s.P, s.Q, s.image, s.itmp, s.w, s.h, s.dt, ws);
+    const dim3 grid(d.ntiles);
 
     if (ev) PFS_HIP_TRY(hipEventRecord(ev[0], stream));
-    for (int done = 0, chunk = 0; done < K; ++chunk) {
-        const int ns = (K - done) < kMaxChunk ? (K - done) : kMaxChunk;
-        const bool last = done + ns == K;
-        const int H = ns + (last ? 1 : 0);
-        d.first = done == 0;
-        d.dsrc = dpl[(chunk + 1) & 1];
-        d.ddst = dpl[chunk & 1];
-        hipLaunchKernelGGL(diffuse_for(H, last), dim3(ntiles), dim3(threads_for(H)), 0, stream,
-                           d);
+    for (int c = 0; c < nchunks; ++c) {
+        diffuse_chunk(d, c, plan[c], ws);
+        hipLaunchKernelGGL(diffuse_for(plan[c].H, plan[c].last), grid,
+                           dim3(threads_for(plan[c].H)), 0, stream, d);
         PFS_TRY(check_launch("fused diffuse"));
-        done += ns;
     }
     if (ev) PFS_HIP_TRY(hipEventRecord(ev[1], stream));
-
-    PressureArgs p{};
-    p.P = s.P;
-    p.Q = s.Q;
-    p.pw = pw;
-    p.bpl = bpl;
-    p.image = s.image;
-    p.itmp = s.itmp;
-    p.w = w;
-    p.h = h;
-    p.tiles_x = tiles_x;
-    p.ntiles = ntiles;
-    p.dt = s.dt;
-    p.fw = (float)w;
-    p.fh = (float)h;
-    p.inv_w = p2w.inv;
-    p.inv_h = p2h.inv;
-    p.pow2_w = p2w.is;
-    p.pow2_h = p2h.is;
-    const bool dye = s.image != nullptr;
-    if (dye) {
-        const ColorConsts cc(w, h, w, h, s.dt);
-        p.dt_viw = cc.dt_viw;
-        p.dt_vih = cc.dt_vih;
-    }
     if (ev) PFS_HIP_TRY(hipEventRecord(ev[2], stream));
-    for (int done = 0, chunk = 0; done < K; ++chunk) {
-        const int ns = (K - done) < kMaxChunk ? (K - done) : kMaxChunk;
-        const bool last = done + ns == K;
-        const int H = ns + (last ? 1 : 0);
-        p.first = done == 0;
-        p.psrc = ppl[(chunk + 1) & 1];
-        p.pdst = ppl[chunk & 1];
-        hipLaunchKernelGGL(pressure_for(H, last, dye && last), dim3(ntiles), dim3(threads_for(H)),
-                           0, stream, p);
+    for (int c = 0; c < nchunks; ++c) {
+        pressure_chunk(p, c, plan[c], ws, s.image, s.itmp);
+        hipLaunchKernelGGL(pressure_for(plan[c].H, plan[c].last), grid,
+                           dim3(threads_for(plan[c].H)), 0, stream, p);
         PFS_TRY(check_launch("fused pressure"));
-        done += ns;
     }
     if (ev) PFS_HIP_TRY(hipEventRecord(ev[3], stream));
-    (void)tiles_y;
     return PFS_OK;
 }
 
