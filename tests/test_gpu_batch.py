@@ -117,3 +117,51 @@ def test_batch_workspace_and_errors():
         b.remove(17)
     b.close()
     np.testing.assert_equal(vp.data.shape, (64, 64, 4))
+
+
+def test_server_continuous_batching_matches_alone():
+    """Jobs arriving over time, admitted into a running batch and retired as
+    they finish, give the same dye as each job run alone."""
+    from probabilistic_fluid_simulation_amd import serve
+
+    specs = scenes.served_specs(12, rate=400.0, seed=7)
+    for sp in specs:  # keep the test small: sides 16..96, 6..40 steps
+        side = 16 + sp["fw"] % 81
+        sp.update(fw=side, fh=side, iw=side, ih=side, steps=6 + sp["steps"] % 35)
+    specs[3].update(iw=specs[3]["fw"] + 5)  # one dye of another size: separate dye kernel
+    jobs = serve.jobs_from_specs(specs, iters=4)
+    srv = serve.Server(max_substeps=4, keep_results=True)
+    out = srv.run(jobs)
+    srv.close()
+    assert out["completed"] == len(jobs)
+    assert out["p99_latency_s"] >= out["p50_latency_s"] > 0
+    for j in jobs:
+        alone = Batch()
+        bufs = serve.device_scene(j)
+        slot = alone.add(*bufs, j.dt, j.visc, j.iters)
+        alone.step(j.steps)
+        alone.sync()
+        assert torch.equal(alone.image(slot), j.result), j.job
+        alone.close()
+
+
+def test_server_admission_respects_budget():
+    from probabilistic_fluid_simulation_amd import serve
+
+    specs = scenes.served_specs(6, rate=1000.0, seed=3)
+    for sp in specs:
+        sp.update(fw=64, fh=64, iw=64, ih=64, steps=8)
+    jobs = serve.jobs_from_specs(specs)
+    one = jobs[0].nbytes
+    srv = serve.Server(hbm_budget=2 * one, max_substeps=2)
+    peak = []
+    orig = srv._admit
+
+    def admit(job, now):
+        orig(job, now)
+        peak.append(len(srv.active))
+
+    srv._admit = admit
+    out = srv.run(jobs)
+    srv.close()
+    assert out["completed"] == 6 and max(peak) <= 2
