@@ -38,6 +38,19 @@ def algorithmic_bytes(n_cells: int, n_pixels: int, k: int) -> dict:
             "pressure_dye": (20 + 12 * k) * n_cells + 32 * n_pixels + 8 * n_cells}
 
 
+def pmc_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (calibrated against a copy of known size),
+    or None if there is none."""
+    path = os.path.join(ROOT, "profiles", "plain-1024x1024-K8_pmc.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(f"traffic_bytes_per_launch k_{kernel} = "):
+            return int(line.rsplit("=", 1)[1])
+    return None
+
+
 def cpu_baseline(scene, budget_steps: int) -> dict:
     import oracle
 
@@ -221,7 +234,10 @@ def main() -> int:
                    "entry_point": "pfs_fluid_timestep (simulate_fluid_step + advect_color_step)",
                    "two_call_ms_per_step": wall_two / args.steps * 1e3},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": pmc_traffic(dom),
+                     "traffic_source": "profiles/plain-1024x1024-K8_pmc.md (rocprofv3 FETCH_SIZE x2 + "
+                                       "WRITE_SIZE x1, calibrated on a 256 MiB copy)",
                      "algorithmic_bytes_per_launch": alg[dom]},
     }
     if not args.no_cpu_baseline:
