@@ -125,6 +125,52 @@ def batched_leg(n_scenes: int, steps: int) -> dict:
             "entry_point": "pfs_batch_step (hipGraph replay)"}
 
 
+def precision_leg(steps: int, sample_every: int, side: int = 2048) -> dict:
+    """The precision workload: side x side field with the 16K workload's
+    generator (4 Taylor-Green cells, seed 0, dt = 1/60, nu = 0.1), at most
+    16 sweeps with early exit at ||r|| <= 1e-4 ||b||, compensated sums.
+    Every `sample_every` steps, the state before the step is also advanced
+    by the FP64 twin (same sweep counts) and by the plain FP32 step; the
+    reported errors are max |x - x64| / max |x64| over u, v."""
+    import numpy as np
+    import torch
+
+    from probabilistic_fluid_simulation_amd import fluid, scenes
+
+    s = scenes.make_scene(side, side, 4, 4, kind="tg", seed=0, dt=1.0 / 60.0, visc=0.1,
+                          iters=16)
+    vp, tmp = fluid.VPField.from_numpy(s.vp), fluid.VPField.from_numpy(s.tmp)
+    iters, rel_res, err_c, err_p = [], [], [], []
+    t_total = 0.0
+    for k in range(steps):
+        if k % sample_every == 0:
+            before = (vp.data.clone(), tmp.data.clone())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rep = fluid.simulate_fluid_step_precise(vp, tmp, s.dt, s.visc, max_iters=16, tol=1e-4)
+        t_total += time.perf_counter() - t0
+        iters.append(rep["iters_used"])
+        rel_res.append(rep["residual_norm"] / rep["b_norm"] if rep["b_norm"] else 0.0)
+        if k % sample_every == 0:
+            v64, t64 = before[0].double(), before[1].double()
+            fluid.simulate_fluid_step_f64(v64, t64, s.dt, s.visc, 16, rep["iters_used"])
+            pv, pt = fluid.VPField(before[0].clone()), fluid.VPField(before[1].clone())
+            fluid.simulate_fluid_step(pv, pt, s.dt, s.visc, rep["iters_used"])
+            ref = v64[..., :2]
+            scale = ref.abs().amax(dim=(0, 1))
+            err_c.append(float(((vp.data[..., :2].double() - ref).abs().amax(dim=(0, 1)) / scale).max()))
+            err_p.append(float(((pv.data[..., :2].double() - ref).abs().amax(dim=(0, 1)) / scale).max()))
+    cells = side * side
+    return {"workload": f"precision-{side}x{side}-K<=16-tol1e-4", "cells": cells, "steps": steps,
+            "cell_steps_per_s": cells * steps / t_total,
+            "iters_used_min": min(iters), "iters_used_max": max(iters),
+            "final_rel_residual_max": max(rel_res),
+            "velocity_rel_err_vs_fp64_max": max(err_c),
+            "plain_fp32_velocity_rel_err_vs_fp64_max": max(err_p),
+            "error_samples": len(err_c),
+            "note": "each step synchronises to read its residual report"}
+
+
 def served_leg(rate: float, duration: float, seed: int = 7) -> dict:
     """The served workload: Poisson arrivals at `rate` jobs/s for `duration`
     seconds (seed 7), dye pixels log-uniform in [8192, 2^20], field side =
@@ -153,6 +199,8 @@ def main() -> int:
     ap.add_argument("--batched-scenes", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=300)
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--precision-steps", type=int, default=20)
     ap.add_argument("--served-rates", default="64",
                     help="comma-separated arrival rates (jobs/s) of the served leg")
     ap.add_argument("--served-seconds", type=float, default=6.0)
@@ -248,6 +296,8 @@ def main() -> int:
     if not args.no_served:
         out["served"] = [served_leg(float(r), args.served_seconds)
                          for r in args.served_rates.split(",")]
+    if not args.no_precision:
+        out["precision"] = precision_leg(args.precision_steps, 5)
     print(json.dumps(out), flush=True)
     if args.copy_probe_mib > 0:
         n = args.copy_probe_mib * (1 << 20) // 4

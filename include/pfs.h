@@ -180,6 +180,38 @@ size_t pfs_batch_workspace_bytes(pfs_batch *b);
 int pfs_batch_set_graphs(pfs_batch *b, int enable);
 void *pfs_batch_stream(pfs_batch *b);
 
+/* ---- precision form --------------------------------------------------------
+ * No counterpart in the reference, whose Jacobi loops run a fixed count in
+ * FP32 (src/fluid.cpp:151,239).  Same step and buffer contract as
+ * pfs_simulate_fluid_step with K = max_iters diffusion sweeps, except:
+ * compensated (twice-FP32) stencil sums in both sweeps; an FP64 residual
+ * ||b + pL + pR + pT + pB - 4p|| of every pressure iterate; and the pressure
+ * iteration stops at the first iterate with residual <= tol * ||b|| (Kp <=
+ * max_iters sweeps), decided on the device.  With Kp = K the result has the
+ * reference's layout: tmp = (E, p^K, b), vp = (E - grad p^K, p^(K-1), b). */
+#define PFS_PRECISE_MAX_ITERS 64
+typedef struct pfs_precise_report {
+    int iters_used;                                    /* Kp */
+    double b_norm;                                     /* ||b||_2 */
+    double residual_norm;                              /* ||r(p^Kp)||_2 */
+    double residual_history[PFS_PRECISE_MAX_ITERS + 1]; /* ||r(p^s)||, s = 0.. */
+} pfs_precise_report;
+
+/* Device workspace bytes for a vx x vy field (fp64 = 1: for the FP64 twin). */
+size_t pfs_precise_workspace_bytes(int vx, int vy, int fp64);
+/* report: if non-NULL, filled after the stream is synchronised. */
+int pfs_simulate_fluid_step_precise(pfs_field *vp, pfs_field *tmp, float dt, float viscosity,
+                                    int max_iters, double tol, void *workspace,
+                                    size_t workspace_bytes, void *stream,
+                                    pfs_precise_report *report);
+/* FP64 twin: the same step and contract on double [vy][vx][4] device buffers
+ * with plain arithmetic and given sweep counts (1 <= iters_pressure <=
+ * iters_diffuse); with equal counts it is the reference's algorithm in
+ * double.  The yardstick the precision form's error is measured against. */
+int pfs_simulate_fluid_step_f64(double *vp, double *tmp, int vx, int vy, double dt,
+                                double viscosity, int iters_diffuse, int iters_pressure,
+                                void *workspace, size_t workspace_bytes, void *stream);
+
 /* ---- debug: the integer "neighbour lists" ------------------------------
  * Back-trace cell keys of advect (src/fluid.cpp:55-57): keys[(j*w+i)*4 +
  * {0,1,2,3}] = {i0, j0, i1, j1} for every cell.  `keys` is a device int32

@@ -20,6 +20,7 @@ PFS_ERR_NOMEM = -3
 PFS_ERR_UNSUPPORTED = -4
 PFS_PATH_AUTO = 0
 PFS_PATH_PER_OP = 1
+PFS_PRECISE_MAX_ITERS = 64
 
 _ERRNAMES = {PFS_ERR_INVALID: "invalid argument", PFS_ERR_HIP: "HIP error",
              PFS_ERR_NOMEM: "out of device memory", PFS_ERR_UNSUPPORTED: "unsupported"}
@@ -29,6 +30,13 @@ class PfsField(ctypes.Structure):
     """pfs_field / vp_field: {int x, y, z; float *data} (includes/fluid.hpp:17-22)."""
     _fields_ = [("x", ctypes.c_int), ("y", ctypes.c_int), ("z", ctypes.c_int),
                 ("data", ctypes.c_void_p)]
+
+
+class PreciseReport(ctypes.Structure):
+    """pfs_precise_report (include/pfs.h)."""
+    _fields_ = [("iters_used", ctypes.c_int), ("b_norm", ctypes.c_double),
+                ("residual_norm", ctypes.c_double),
+                ("residual_history", ctypes.c_double * (PFS_PRECISE_MAX_ITERS + 1))]
 
 
 class PfsError(RuntimeError):
@@ -73,12 +81,18 @@ SIGNATURES = {
     "pfs_batch_workspace_bytes": [_V],
     "pfs_batch_set_graphs": [_V, _I],
     "pfs_batch_stream": [_V],
+    "pfs_precise_workspace_bytes": [_I, _I, _I],
+    "pfs_simulate_fluid_step_precise": [_P, _P, _F, _F, _I, ctypes.c_double, _V, ctypes.c_size_t,
+                                        _V, ctypes.POINTER(PreciseReport)],
+    "pfs_simulate_fluid_step_f64": [_V, _V, _I, _I, ctypes.c_double, ctypes.c_double, _I, _I, _V,
+                                    ctypes.c_size_t, _V],
     "pfs_version": [],
     "pfs_last_error": [],
 }
 _RESTYPES = {"pfs_version": ctypes.c_char_p, "pfs_last_error": ctypes.c_char_p,
              "pfs_step_workspace_bytes": ctypes.c_size_t,
-             "pfs_batch_workspace_bytes": ctypes.c_size_t, "pfs_batch_stream": ctypes.c_void_p}
+             "pfs_batch_workspace_bytes": ctypes.c_size_t, "pfs_batch_stream": ctypes.c_void_p,
+             "pfs_precise_workspace_bytes": ctypes.c_size_t}
 
 _lock = threading.Lock()
 _lib = None

@@ -10,52 +10,62 @@
 
 namespace pfs {
 
-// fmodf(fmodf(x, n) + n, n) for an integer-valued n > 0 (src/fluid.cpp:48-49,
-// 105-106), exactly.  When |x| < n, fmodf(x, n) == x; x + n is rounded once
-// to s in [0, 2n] and fmodf(s, n) is s, s - n (exact by Sterbenz' lemma for
-// s in [n, 2n)) or 0 for s == 2n.  Other inputs take the libm-exact fmodf.
-__device__ __forceinline__ float wrap_coord(float x, float n) {
-    if (fabsf(x) < n) {
-        float s = x + n;
+// IEEE fmod in the field's precision (fmodf for float, as the reference's
+// C++ overload resolution picks for float arguments, src/fluid.cpp:48).
+__device__ __forceinline__ float fmod_t(float x, float y) { return fmodf(x, y); }
+__device__ __forceinline__ double fmod_t(double x, double y) { return fmod(x, y); }
+
+// fmod(fmod(x, n) + n, n) for an integer-valued n > 0 (src/fluid.cpp:48-49,
+// 105-106), exactly.  When |x| < n, fmod(x, n) == x; x + n is rounded once
+// to s in [0, 2n] and fmod(s, n) is s, s - n (exact by Sterbenz' lemma for
+// s in [n, 2n)) or 0 for s == 2n.  Other inputs take the exact libm fmod.
+template <typename T>
+__device__ __forceinline__ T wrap_coord(T x, T n) {
+    if (fabs(x) < n) {
+        T s = x + n;
         s = (s >= n) ? s - n : s;
         return (s >= n) ? s - n : s;
     }
-    return fmodf(fmodf(x, n) + n, n);
+    return fmod_t(fmod_t(x, n) + n, n);
 }
 
 // x / n, correctly rounded.  For n a power of two the product with 1/n is the
 // same correctly rounded value of the same real number.
-__device__ __forceinline__ float div_n(float x, float n, float inv_n, bool n_pow2) {
+template <typename T>
+__device__ __forceinline__ T div_n(T x, T n, T inv_n, bool n_pow2) {
     return n_pow2 ? x * inv_n : x / n;
 }
 
 // Bilinear blend in the reference's exact order (src/fluid.cpp:62-66).
-__device__ __forceinline__ float bilerp(float sx, float sy, float f00, float f10, float f01,
-                                       float f11) {
+template <typename T>
+__device__ __forceinline__ T bilerp(T sx, T sy, T f00, T f10, T f01, T f11) {
     return (1 - sx) * (1 - sy) * f00 + sx * (1 - sy) * f10 + (1 - sx) * sy * f01 +
            sx * sy * f11;
 }
 
 // Cell keys of one back-trace (src/fluid.cpp:48-58).
-struct Trace {
+template <typename T>
+struct TraceT {
     int i0, j0, i1, j1;
-    float sx, sy;
+    T sx, sy;
 };
+using Trace = TraceT<float>;
 
 // Wraps (xp, yp) into [0, w) x [0, h) and returns the corner keys and weights.
 // The clamp of i0/j0 is a no-op for every finite input (the wrapped
 // coordinate lies in [0, n)); it only keeps a NaN/Inf velocity, for which
 // the reference indexes memory out of bounds, inside the field.
-__device__ __forceinline__ Trace trace(float xp, float yp, int w, int h, float fw, float fh) {
+template <typename T>
+__device__ __forceinline__ TraceT<T> trace(T xp, T yp, int w, int h, T fw, T fh) {
     xp = wrap_coord(xp, fw);
     yp = wrap_coord(yp, fh);
-    Trace t;
+    TraceT<T> t;
     t.i0 = min(max((int)xp, 0), w - 1);
     t.j0 = min(max((int)yp, 0), h - 1);
     t.i1 = (t.i0 + 1 == w) ? 0 : t.i0 + 1;  // (i0 + 1) % w for i0 in [0, w)
     t.j1 = (t.j0 + 1 == h) ? 0 : t.j0 + 1;
-    t.sx = xp - (float)t.i0;
-    t.sy = yp - (float)t.j0;
+    t.sx = xp - (T)t.i0;
+    t.sy = yp - (T)t.j0;
     return t;
 }
 
@@ -64,9 +74,9 @@ __device__ __forceinline__ int wrap_minus(int i, int n) { return i == 0 ? n - 1 
 __device__ __forceinline__ int wrap_plus(int i, int n) { return i + 1 == n ? 0 : i + 1; }
 
 // Diffusion sweep of one channel (src/fluid.cpp:19-21,168-182).
-__device__ __forceinline__ float diffuse_cell(float l, float r, float t, float b, float c,
-                                              float alpha, float beta) {
-    return (alpha * l + alpha * r + alpha * t + alpha * b + 1.0f * c) / beta;
+template <typename T>
+__device__ __forceinline__ T diffuse_cell(T l, T r, T t, T b, T c, T alpha, T beta) {
+    return (alpha * l + alpha * r + alpha * t + alpha * b + (T)1.0f * c) / beta;
 }
 
 // x / beta correctly rounded, for a divisor fixed per launch, with y =
@@ -108,8 +118,9 @@ __device__ __forceinline__ float div_fixed(float x, float beta, float y, bool fa
 }
 
 // Pressure sweep (src/fluid.cpp:249-255): jacobi(pL, pR, pT, pB, 1, 4, b).
-__device__ __forceinline__ float pressure_cell(float l, float r, float t, float b, float rhs) {
-    return (l + r + t + b + 1.0f * rhs) / 4.0f;
+template <typename T>
+__device__ __forceinline__ T pressure_cell(T l, T r, T t, T b, T rhs) {
+    return (l + r + t + b + (T)1.0f * rhs) / (T)4.0f;
 }
 
 // Constants of advect_color (src/fluid.cpp:78-83,97-98), computed on the

@@ -161,6 +161,52 @@ def fluid_timestep(vp: VPField, tmp: VPField, image: VPField, itmp: VPField, dt:
     b.sync_back()
 
 
+_precise_ws = {}
+
+
+def _workspace(key, nbytes: int, device) -> torch.Tensor:
+    ws = _precise_ws.get(key)
+    if ws is None or ws.numel() < nbytes or ws.device != device:
+        ws = torch.empty(nbytes, dtype=torch.uint8, device=device)
+        _precise_ws[key] = ws
+    return ws
+
+
+def simulate_fluid_step_precise(vp: VPField, tmp: VPField, dt: float, viscosity: float,
+                                max_iters: int = 16, tol: float = 1e-4, stream=None) -> dict:
+    """The precision form (no reference counterpart): simulate_fluid_step
+    with compensated stencil sums, an FP64 pressure residual per sweep and
+    early exit at ||r|| <= tol ||b|| (at most max_iters sweeps).  Returns the
+    report (synchronises the stream)."""
+    lib = _lib.load()
+    nbytes = lib.pfs_precise_workspace_bytes(vp.x, vp.y, 0)
+    ws = _workspace(("f32", vp.data.device), nbytes, vp.data.device)
+    rep = _lib.PreciseReport()
+    b = _Bound(vp, tmp)
+    _lib.call("pfs_simulate_fluid_step_precise", b.ptr(0), b.ptr(1), dt, viscosity, max_iters,
+              tol, ctypes.c_void_p(ws.data_ptr()), ws.numel(), _stream(stream), ctypes.byref(rep))
+    return {"iters_used": rep.iters_used, "b_norm": rep.b_norm,
+            "residual_norm": rep.residual_norm,
+            "residual_history": list(rep.residual_history)[:max(rep.iters_used + 2, 1)]}
+
+
+def simulate_fluid_step_f64(vp: torch.Tensor, tmp: torch.Tensor, dt: float, viscosity: float,
+                            iters_diffuse: int, iters_pressure: int, stream=None) -> None:
+    """FP64 twin of the step on (H, W, 4) float64 device tensors: plain double
+    arithmetic, given sweep counts, the precision form's buffer contract."""
+    if vp.dtype != torch.float64 or tmp.dtype != torch.float64 or vp.shape != tmp.shape:
+        raise ValueError("vp and tmp must be float64 tensors of one shape (H, W, 4)")
+    h, w, c = vp.shape
+    if c != 4 or not (vp.is_contiguous() and tmp.is_contiguous()):
+        raise ValueError("need contiguous (H, W, 4) tensors")
+    lib = _lib.load()
+    nbytes = lib.pfs_precise_workspace_bytes(w, h, 1)
+    ws = _workspace(("f64", vp.device), nbytes, vp.device)
+    _lib.call("pfs_simulate_fluid_step_f64", ctypes.c_void_p(vp.data_ptr()),
+              ctypes.c_void_p(tmp.data_ptr()), w, h, dt, viscosity, iters_diffuse,
+              iters_pressure, ctypes.c_void_p(ws.data_ptr()), ws.numel(), _stream(stream))
+
+
 # --- library controls -------------------------------------------------------------
 def set_step_path(path: str) -> None:
     """"auto" (fused gfx950 kernels) or "per_op" (one launch per operator/sweep)."""
