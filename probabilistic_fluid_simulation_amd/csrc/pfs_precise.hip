@@ -42,6 +42,7 @@ namespace pfs {
 namespace {
 
 constexpr int kBX = 64, kBY = 4;
+constexpr int kRows = 8;  // rows per thread: a block covers kBX x (kBY * kRows) cells
 constexpr int kMaxK = PFS_PRECISE_MAX_ITERS;
 
 template <typename T>
@@ -55,13 +56,34 @@ struct V2<double> {
     using type = double2;
 };
 
+// FP64 sums are accumulated with one atomic per block, spread over kSlots
+// addresses a cache line apart (slot = block index mod kSlots) so that
+// blocks do not all queue on one address; a reader adds the slots in
+// slot order.
+constexpr int kSlots = 32;
+constexpr int kSlotStride = 16;     // doubles: 128 bytes
+constexpr int kSumB2 = kMaxK + 1;   // row of sum b^2
+constexpr int kSumFinal = kMaxK + 2;  // row of sum r^2 of the kept iterate
+constexpr int kSums = kMaxK + 3;    // rows 0..kMaxK: sum r^2 of pressure iterate s
+
 // Device-side state of one precision step (cleared by k_ctl_clear).
 struct Ctl {
-    double acc[kMaxK + 1];  // acc[s] = sum r^2 of pressure iterate s
-    double b2;              // sum b^2
-    double final_r2;        // sum r^2 of the iterate the step kept
+    double sum[kSums][kSlots * kSlotStride];
     int done, kp;
 };
+
+// The folded totals the host reads back (written by k_ctl_fold).
+struct CtlTotals {
+    double sum[kSums];
+    int kp, pad;
+};
+
+__device__ __forceinline__ double ctl_total(const Ctl *c, int row) {
+    double t = 0.0;
+#pragma unroll 8
+    for (int k = 0; k < kSlots; ++k) t += c->sum[row][k * kSlotStride];
+    return t;
+}
 
 // ---- compensated arithmetic (error-free transformations) -------------------
 __device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
@@ -104,8 +126,9 @@ __device__ __forceinline__ float pressure_sum_comp(float l, float r, float t, fl
     return s + err;
 }
 
-// Block-wide FP64 sum into *dst (one atomic per block).
-__device__ __forceinline__ void block_add(double v, double *dst) {
+// Block-wide FP64 sum into row `row` of the control sums (one atomic per
+// block).
+__device__ __forceinline__ void block_add(double v, Ctl *ctl, int row) {
     __shared__ double part[kBX * kBY / 64];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -115,17 +138,14 @@ __device__ __forceinline__ void block_add(double v, double *dst) {
     if (tid == 0) {
         double s = 0.0;
         for (int k = 0; k < kBX * kBY / 64; ++k) s += part[k];
-        atomicAdd(dst, s);
+        const int slot = (int)((blockIdx.y * gridDim.x + blockIdx.x) % kSlots);
+        atomicAdd(&ctl->sum[row][slot * kSlotStride], s);
     }
 }
 
 struct Cell {
     int i, j, im, ip, jm, jp;
-    bool ok;
-    __device__ Cell(int w, int h) {
-        i = blockIdx.x * kBX + threadIdx.x;
-        j = blockIdx.y * kBY + threadIdx.y;
-        ok = i < w && j < h;
+    __device__ Cell(int w, int h, int i_, int j_) : i(i_), j(j_) {
         im = wrap_minus(i, w);
         ip = wrap_plus(i, w);
         jm = wrap_minus(j, h);
@@ -133,14 +153,34 @@ struct Cell {
     }
 };
 
+// Calls f(cell) for each of this thread's cells: column blockIdx.x*kBX + x,
+// rows kRows apart by kBY within the block's kBY*kRows rows.
+template <typename F>
+__device__ __forceinline__ void for_cells(int w, int h, F &&f) {
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j0 = blockIdx.y * (kBY * kRows) + threadIdx.y;
+    if (i >= w) return;
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+        const int j = j0 + r * kBY;
+        if (j < h) f(Cell(w, h, i, j));
+    }
+}
+
+// Grid (kSums): block r clears row r; block 0 also the decision.
 __global__ void k_ctl_clear(Ctl *c) {
-    for (int k = threadIdx.x; k <= kMaxK; k += blockDim.x) c->acc[k] = 0.0;
-    if (threadIdx.x == 0) {
-        c->b2 = 0.0;
-        c->final_r2 = 0.0;
+    for (int k = threadIdx.x; k < kSlots * kSlotStride; k += blockDim.x) c->sum[blockIdx.x][k] = 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         c->done = 0;
         c->kp = 0;
     }
+}
+
+// One block of 128 threads: thread r < kSums folds row r.
+static_assert(kSums <= 128, "k_ctl_fold folds one row per thread");
+__global__ void k_ctl_fold(const Ctl *c, CtlTotals *t) {
+    if ((int)threadIdx.x < kSums) t->sum[threadIdx.x] = ctl_total(c, threadIdx.x);
+    if (threadIdx.x == 0) t->kp = c->kp;
 }
 
 // Advection of vp[0,1] (src/fluid.cpp:24-70) into the iterate plane d.
@@ -149,16 +189,16 @@ __global__ __launch_bounds__(kBX *kBY) void k_p_advect(const T *__restrict__ P,
                                                       typename V2<T>::type *__restrict__ d, int w,
                                                       int h, T dt, T fw, T fh, T inv_w, T inv_h,
                                                       int pow2_w, int pow2_h) {
-    const Cell c(w, h);
-    if (!c.ok) return;
-    const int cell = c.j * w + c.i;
-    const T xp = (T)c.i - div_n(dt * P[4 * cell], fw, inv_w, pow2_w);
-    const T yp = (T)c.j - div_n(dt * P[4 * cell + 1], fh, inv_h, pow2_h);
-    const TraceT<T> t = trace(xp, yp, w, h, fw, fh);
-    const int a00 = 4 * (t.j0 * w + t.i0), a10 = 4 * (t.j0 * w + t.i1);
-    const int a01 = 4 * (t.j1 * w + t.i0), a11 = 4 * (t.j1 * w + t.i1);
-    d[cell] = {bilerp(t.sx, t.sy, P[a00], P[a10], P[a01], P[a11]),
-               bilerp(t.sx, t.sy, P[a00 + 1], P[a10 + 1], P[a01 + 1], P[a11 + 1])};
+    for_cells(w, h, [&](const Cell &c) {
+        const int cell = c.j * w + c.i;
+        const T xp = (T)c.i - div_n(dt * P[4 * cell], fw, inv_w, pow2_w);
+        const T yp = (T)c.j - div_n(dt * P[4 * cell + 1], fh, inv_h, pow2_h);
+        const TraceT<T> t = trace(xp, yp, w, h, fw, fh);
+        const int a00 = 4 * (t.j0 * w + t.i0), a10 = 4 * (t.j0 * w + t.i1);
+        const int a01 = 4 * (t.j1 * w + t.i0), a11 = 4 * (t.j1 * w + t.i1);
+        d[cell] = {bilerp(t.sx, t.sy, P[a00], P[a10], P[a01], P[a11]),
+                   bilerp(t.sx, t.sy, P[a00 + 1], P[a10 + 1], P[a01 + 1], P[a11 + 1])};
+    });
 }
 
 // One diffusion sweep (src/fluid.cpp:154-185) on the iterate planes.
@@ -166,17 +206,17 @@ template <typename T, bool COMP>
 __global__ __launch_bounds__(kBX *kBY) void k_p_diffuse(const typename V2<T>::type *__restrict__ in,
                                                        typename V2<T>::type *__restrict__ out,
                                                        int w, int h, T alpha, T beta) {
-    const Cell c(w, h);
-    if (!c.ok) return;
-    const auto L = in[c.j * w + c.im], R = in[c.j * w + c.ip], Tp = in[c.jm * w + c.i],
-               B = in[c.jp * w + c.i], C = in[c.j * w + c.i];
-    if constexpr (COMP) {
-        out[c.j * w + c.i] = {diffuse_sum_comp(L.x, R.x, Tp.x, B.x, C.x, alpha) / beta,
-                              diffuse_sum_comp(L.y, R.y, Tp.y, B.y, C.y, alpha) / beta};
-    } else {
-        out[c.j * w + c.i] = {diffuse_cell(L.x, R.x, Tp.x, B.x, C.x, alpha, beta),
-                              diffuse_cell(L.y, R.y, Tp.y, B.y, C.y, alpha, beta)};
-    }
+    for_cells(w, h, [&](const Cell &c) {
+        const auto L = in[c.j * w + c.im], R = in[c.j * w + c.ip], Tp = in[c.jm * w + c.i],
+                   B = in[c.jp * w + c.i], C = in[c.j * w + c.i];
+        if constexpr (COMP) {
+            out[c.j * w + c.i] = {diffuse_sum_comp(L.x, R.x, Tp.x, B.x, C.x, alpha) / beta,
+                                  diffuse_sum_comp(L.y, R.y, Tp.y, B.y, C.y, alpha) / beta};
+        } else {
+            out[c.j * w + c.i] = {diffuse_cell(L.x, R.x, Tp.x, B.x, C.x, alpha, beta),
+                                  diffuse_cell(L.y, R.y, Tp.y, B.y, C.y, alpha, beta)};
+        }
+    });
 }
 
 // Divergence of D^Kd (src/fluid.cpp:221-237) into b, and the pressure start
@@ -187,22 +227,21 @@ __global__ __launch_bounds__(kBX *kBY) void k_p_divergence(const typename V2<T>:
                                                           const T *__restrict__ Q, T *__restrict__ b,
                                                           T *__restrict__ p0, int w, int h,
                                                           T gamma, int kd_even, Ctl *ctl) {
-    const Cell c(w, h);
     double b2 = 0.0;
-    if (c.ok) {
+    for_cells(w, h, [&](const Cell &c) {
         const int cell = c.j * w + c.i;
         const T uR = dk[c.j * w + c.ip].x - dk[c.j * w + c.im].x;
         const T vT = dk[c.jp * w + c.i].y - dk[c.jm * w + c.i].y;
         const T bb = gamma * (uR + vT);
         b[cell] = bb;
         p0[cell] = kd_even ? Q[4 * cell + 2] : P[4 * cell + 2];
-        b2 = (double)bb * (double)bb;
-    }
-    if constexpr (COMP) block_add(b2, &ctl->b2);
+        b2 += (double)bb * (double)bb;
+    });
+    if constexpr (COMP) block_add(b2, ctl, kSumB2);
 }
 
 // Pressure sweep s (src/fluid.cpp:240-257): p^s from p^(s-1).  With COMP,
-// also the FP64 residual of p^(s-1) into acc[s-1], and the convergence test
+// also the FP64 residual of p^(s-1) into row s-1, and the convergence test
 // of iterate s-2, whose residual the previous sweep completed.
 template <typename T, bool COMP>
 __global__ __launch_bounds__(kBX *kBY) void k_p_pressure(const T *__restrict__ in,
@@ -211,10 +250,11 @@ __global__ __launch_bounds__(kBX *kBY) void k_p_pressure(const T *__restrict__ i
                                                         double tol2, Ctl *ctl) {
     if constexpr (COMP) {
         // One thread decides for the block; every block reaches the same
-        // verdict from the same completed sums.
+        // verdict from the same completed sums, added in the same order.
         __shared__ int skip;
         if (threadIdx.x == 0 && threadIdx.y == 0) {
-            const bool converged = s >= 3 && ctl->acc[s - 2] <= tol2 * ctl->b2;
+            const bool converged =
+                s >= 3 && ctl_total(ctl, s - 2) <= tol2 * ctl_total(ctl, kSumB2);
             skip = ctl->done || converged;
             if (converged && !ctl->done) {
                 ctl->kp = s - 2;
@@ -224,9 +264,8 @@ __global__ __launch_bounds__(kBX *kBY) void k_p_pressure(const T *__restrict__ i
         __syncthreads();
         if (skip) return;
     }
-    const Cell c(w, h);
     double r2 = 0.0;
-    if (c.ok) {
+    for_cells(w, h, [&](const Cell &c) {
         const int cell = c.j * w + c.i;
         const T pL = in[c.j * w + c.im], pR = in[c.j * w + c.ip];
         const T pT = in[c.jm * w + c.i], pB = in[c.jp * w + c.i];
@@ -235,12 +274,12 @@ __global__ __launch_bounds__(kBX *kBY) void k_p_pressure(const T *__restrict__ i
             out[cell] = pressure_sum_comp(pL, pR, pT, pB, bb) * 0.25f;
             const double r = (double)bb + (double)pL + (double)pR + (double)pT + (double)pB -
                              4.0 * (double)in[cell];
-            r2 = r * r;
+            r2 += r * r;
         } else {
             out[cell] = pressure_cell(pL, pR, pT, pB, bb);
         }
-    }
-    if constexpr (COMP) block_add(r2, &ctl->acc[s - 1]);
+    });
+    if constexpr (COMP) block_add(r2, ctl, s - 1);
 }
 
 // Gradient subtraction with p^Kp (src/fluid.cpp:269-296) and the output
@@ -256,16 +295,17 @@ __global__ __launch_bounds__(kBX *kBY) void k_p_finalize(
         __shared__ int kp_s;
         if (threadIdx.x == 0 && threadIdx.y == 0)
             kp_s = ctl->done ? ctl->kp
-                             : ((kd >= 2 && ctl->acc[kd - 1] <= tol2 * ctl->b2) ? kd - 1 : kd);
+                             : ((kd >= 2 && ctl_total(ctl, kd - 1) <= tol2 * ctl_total(ctl, kSumB2))
+                                    ? kd - 1
+                                    : kd);
         __syncthreads();
         kp = kp_s;
     }
     // Iterate k lives in plane k % 3.
     const T *pk = kp % 3 == 0 ? p0 : kp % 3 == 1 ? p1 : p2;
     const T *pkm1 = kp % 3 == 1 ? p0 : kp % 3 == 2 ? p1 : p2;
-    const Cell c(w, h);
     double r2 = 0.0;
-    if (c.ok) {
+    for_cells(w, h, [&](const Cell &c) {
         const int cell = c.j * w + c.i;
         const T pL = pk[c.j * w + c.im], pR = pk[c.j * w + c.ip];
         const T pT = pk[c.jm * w + c.i], pB = pk[c.jp * w + c.i];
@@ -283,10 +323,10 @@ __global__ __launch_bounds__(kBX *kBY) void k_p_finalize(
         Q[4 * cell + 3] = bb;
         const double r = (double)bb + (double)pL + (double)pR + (double)pT + (double)pB -
                          4.0 * (double)pk[cell];
-        r2 = r * r;
-    }
+        r2 += r * r;
+    });
     if constexpr (COMP) {
-        block_add(r2, &ctl->final_r2);
+        block_add(r2, ctl, kSumFinal);
         if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0)
             ctl->kp = kp;
     }
@@ -300,12 +340,13 @@ int precise_step(T *P, T *Q, int w, int h, T dt, T visc, int kd, int kp, double 
     T2 *d[2] = {static_cast<T2 *>(ws), static_cast<T2 *>(ws) + N};
     T *b = reinterpret_cast<T *>(d[1] + N);
     T *p[3] = {b + N, b + 2 * N, b + 3 * N};
-    const dim3 grid((w + kBX - 1) / kBX, (h + kBY - 1) / kBY), block(kBX, kBY);
+    const dim3 grid((w + kBX - 1) / kBX, (h + kBY * kRows - 1) / (kBY * kRows)),
+        block(kBX, kBY);
     const T alpha = visc * dt;
     const T beta = (T)(1.0 + 4.0 * (double)alpha);  // src/fluid.cpp:145
     const T gamma = (T)(-1.0 / (double)dt);         // src/fluid.cpp:218
     const double tol2 = tol * tol;
-    if (COMP) hipLaunchKernelGGL(k_ctl_clear, dim3(1), dim3(256), 0, s, ctl);
+    if (COMP) hipLaunchKernelGGL(k_ctl_clear, dim3(kSums), dim3(256), 0, s, ctl);
     hipLaunchKernelGGL((k_p_advect<T>), grid, block, 0, s, P, d[0], w, h, dt, (T)w, (T)h,
                        (T)1 / (T)w, (T)1 / (T)h, (w & (w - 1)) == 0, (h & (h - 1)) == 0);
     for (int k = 1; k <= kd; ++k)
@@ -322,11 +363,15 @@ int precise_step(T *P, T *Q, int w, int h, T dt, T visc, int kd, int kp, double 
 }
 
 // Workspace: two iterate planes of T2, b and three pressure planes of T,
-// then (FP32 only) the control block, 256-byte aligned.
+// then (FP32 only) the control block and its folded totals, each 256-byte
+// aligned.
+constexpr size_t kCtlBytes = (sizeof(Ctl) + 255) / 256 * 256;
+
 template <typename T>
 size_t precise_ws_bytes(int w, int h) {
     const size_t N = (size_t)w * h;
-    return N * (2 * sizeof(typename V2<T>::type) + 4 * sizeof(T)) + sizeof(Ctl) + 256;
+    return N * (2 * sizeof(typename V2<T>::type) + 4 * sizeof(T)) + kCtlBytes +
+           sizeof(CtlTotals) + 256;
 }
 
 }  // namespace
@@ -361,14 +406,17 @@ int pfs_simulate_fluid_step_precise(pfs_field *vp, pfs_field *tmp, float dt, flo
     PFS_TRY((precise_step<float, true>(vp->data, tmp->data, vp->x, vp->y, dt, viscosity, max_iters,
                                        max_iters, tol, workspace, s, ctl)));
     if (report) {
-        Ctl h;
-        PFS_HIP_TRY(hipMemcpyAsync(&h, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+        CtlTotals *tot = reinterpret_cast<CtlTotals *>(reinterpret_cast<char *>(ctl) + kCtlBytes);
+        hipLaunchKernelGGL(k_ctl_fold, dim3(1), dim3(128), 0, s, ctl, tot);
+        PFS_TRY(check_launch("precision report"));
+        CtlTotals h;
+        PFS_HIP_TRY(hipMemcpyAsync(&h, tot, sizeof(h), hipMemcpyDeviceToHost, s));
         PFS_HIP_TRY(hipStreamSynchronize(s));
         std::memset(report, 0, sizeof(*report));
         report->iters_used = h.kp;
-        report->b_norm = std::sqrt(h.b2);
-        report->residual_norm = std::sqrt(h.final_r2);
-        for (int k = 0; k <= kMaxK; ++k) report->residual_history[k] = std::sqrt(h.acc[k]);
+        report->b_norm = std::sqrt(h.sum[kSumB2]);
+        report->residual_norm = std::sqrt(h.sum[kSumFinal]);
+        for (int k = 0; k <= kMaxK; ++k) report->residual_history[k] = std::sqrt(h.sum[k]);
     }
     return PFS_OK;
 }

@@ -173,18 +173,23 @@ def _workspace(key, nbytes: int, device) -> torch.Tensor:
 
 
 def simulate_fluid_step_precise(vp: VPField, tmp: VPField, dt: float, viscosity: float,
-                                max_iters: int = 16, tol: float = 1e-4, stream=None) -> dict:
+                                max_iters: int = 16, tol: float = 1e-4, stream=None,
+                                report: bool = True) -> Optional[dict]:
     """The precision form (no reference counterpart): simulate_fluid_step
     with compensated stencil sums, an FP64 pressure residual per sweep and
     early exit at ||r|| <= tol ||b|| (at most max_iters sweeps).  Returns the
-    report (synchronises the stream)."""
+    report, which synchronises the stream; report=False queues the step
+    without waiting and returns None."""
     lib = _lib.load()
     nbytes = lib.pfs_precise_workspace_bytes(vp.x, vp.y, 0)
     ws = _workspace(("f32", vp.data.device), nbytes, vp.data.device)
     rep = _lib.PreciseReport()
     b = _Bound(vp, tmp)
     _lib.call("pfs_simulate_fluid_step_precise", b.ptr(0), b.ptr(1), dt, viscosity, max_iters,
-              tol, ctypes.c_void_p(ws.data_ptr()), ws.numel(), _stream(stream), ctypes.byref(rep))
+              tol, ctypes.c_void_p(ws.data_ptr()), ws.numel(), _stream(stream),
+              ctypes.byref(rep) if report else None)
+    if not report:
+        return None
     return {"iters_used": rep.iters_used, "b_norm": rep.b_norm,
             "residual_norm": rep.residual_norm,
             "residual_history": list(rep.residual_history)[:max(rep.iters_used + 2, 1)]}
