@@ -164,7 +164,9 @@ int pfs_batch_destroy(pfs_batch *b);
  * *slot identifies it until pfs_batch_remove. */
 int pfs_batch_add(pfs_batch *b, const pfs_field *vp, const pfs_field *tmp, const pfs_field *image,
                   const pfs_field *itmp, float dt, float viscosity, int iters, int *slot);
-/* Waits for the batch's queued work, then removes the scene. */
+/* Removes the scene from later steps without waiting: steps already queued
+ * still read and write its buffers, so the caller keeps them until
+ * pfs_batch_sync (the batch releases its workspace once those steps end). */
 int pfs_batch_remove(pfs_batch *b, int slot);
 /* Enqueues nsteps timesteps of every scene (simulate_fluid_step +
  * advect_color_step each); returns without waiting. */
