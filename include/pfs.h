@@ -139,11 +139,13 @@ size_t pfs_step_workspace_bytes(int vx, int vy);
 
 /* Per-kernel timing of one fused step (image/itmp NULL) or timestep
  * (pfs_fluid_timestep) on `stream`, measured with HIP events recorded on
- * that stream around each kernel group: ms[0] = advect+diffusion+divergence
- * kernel(s), ms[1] = pressure+projection(+dye) kernel(s).  Runs one real
- * step (the buffers advance).  Synchronises the stream. */
+ * that stream around each kernel group, the group launched `reps` times
+ * back to back (each group is idempotent: it never reads what it writes):
+ * ms[0] = advect+diffusion+divergence kernel(s), ms[1] =
+ * pressure+projection(+dye) kernel(s), per launch of the group.  The
+ * buffers advance by exactly one step.  Synchronises the stream. */
 int pfs_profile_step(pfs_field *vp, pfs_field *tmp, pfs_field *image, pfs_field *itmp, float dt,
-                     float viscosity, int iters, void *stream, float *ms, int n_ms);
+                     float viscosity, int iters, int reps, void *stream, float *ms, int n_ms);
 
 /* ---- batched scenes --------------------------------------------------------
  * No counterpart in the reference, which steps one scene per call.  A batch

@@ -67,7 +67,7 @@ SIGNATURES = {
     "pfs_set_step_path": [_I],
     "pfs_get_step_path": [],
     "pfs_step_workspace_bytes": [_I, _I],
-    "pfs_profile_step": [_P, _P, _P, _P, _F, _F, _I, _V, ctypes.POINTER(ctypes.c_float), _I],
+    "pfs_profile_step": [_P, _P, _P, _P, _F, _F, _I, _I, _V, ctypes.POINTER(ctypes.c_float), _I],
     "pfs_debug_advect_keys": [_P, _F, _V, _V],
     "pfs_debug_advect_color_keys": [_P, _P, _F, _V, _V],
     "pfs_batch_create": [ctypes.POINTER(ctypes.c_void_p), _V],

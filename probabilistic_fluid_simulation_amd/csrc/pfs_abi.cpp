@@ -198,7 +198,7 @@ static bool fused_ok(const pfs_field *vp, const pfs_field *tmp, int iters) {
 // (image, itmp) as one timestep of src/main.cpp:222-225.  `ev` (4 events)
 // requests per-kernel timing of the fused path.
 static int timestep(pfs_field *vp, pfs_field *tmp, pfs_field *image, pfs_field *itmp, float dt,
-                    float viscosity, int iters, void *stream, hipEvent_t *ev) {
+                    float viscosity, int iters, void *stream, hipEvent_t *ev, int reps = 1) {
     PFS_TRY(check_step_fields(vp, tmp, iters));
     if (image) {
         PFS_TRY(check_field(image, "image", 1));
@@ -222,7 +222,7 @@ static int timestep(pfs_field *vp, pfs_field *tmp, pfs_field *image, pfs_field *
     PFS_TRY(get_workspace(fused_workspace_bytes(vp->x, vp->y), (hipStream_t)stream, &ws));
     StepArgs a{vp->data, tmp->data, dye_fused ? image->data : nullptr,
                dye_fused ? itmp->data : nullptr, vp->x, vp->y, dt, viscosity, iters};
-    PFS_TRY(fused_step(a, ws, (hipStream_t)stream, ev));
+    PFS_TRY(fused_step(a, ws, (hipStream_t)stream, ev, reps));
     if (!image) return PFS_OK;
     if (!dye_fused) return pfs_advect_color_step(image, itmp, vp, dt, stream);
     std::swap(image->data, itmp->data);  // src/fluid.cpp:317-319
@@ -282,17 +282,20 @@ size_t pfs_step_workspace_bytes(int vx, int vy) {
 }
 
 int pfs_profile_step(pfs_field *vp, pfs_field *tmp, pfs_field *image, pfs_field *itmp, float dt,
-                     float viscosity, int iters, void *stream, float *ms, int n_ms) {
+                     float viscosity, int iters, int reps, void *stream, float *ms, int n_ms) {
     if (!ms || n_ms < 2) return set_error(PFS_ERR_INVALID, "ms needs room for 2 values");
+    if (reps < 1) return set_error(PFS_ERR_INVALID, "reps must be >= 1");
     if ((image == nullptr) != (itmp == nullptr))
         return set_error(PFS_ERR_INVALID, "image and itmp must both be given or both be NULL");
     hipEvent_t ev[4] = {};
     for (auto &e : ev) PFS_HIP_TRY(hipEventCreate(&e));
-    int rc = timestep(vp, tmp, image, itmp, dt, viscosity, iters, stream, ev);
+    int rc = timestep(vp, tmp, image, itmp, dt, viscosity, iters, stream, ev, reps);
     if (rc == PFS_OK) {
         hipError_t e = hipEventSynchronize(ev[3]);
         if (e == hipSuccess) e = hipEventElapsedTime(&ms[0], ev[0], ev[1]);
         if (e == hipSuccess) e = hipEventElapsedTime(&ms[1], ev[2], ev[3]);
+        ms[0] /= (float)reps;
+        ms[1] /= (float)reps;
         if (e != hipSuccess)
             rc = set_error(PFS_ERR_HIP, std::string("event timing: ") + hipGetErrorString(e));
     }

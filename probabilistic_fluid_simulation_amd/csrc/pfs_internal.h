@@ -85,7 +85,10 @@ struct StepArgs {
 // `ws` is a device workspace of fused_workspace_bytes(w, h).  Optional
 // per-kernel event timing: if `ev` is non-null, events ev[2k] and ev[2k+1]
 // are recorded around kernel group k (0 = diffuse, 1 = pressure).
-int fused_step(const StepArgs &a, float *ws, hipStream_t s, hipEvent_t *ev);
+// ev (profiling): events 0/1 around the diffusion kernels, 2/3 around the
+// pressure kernels, each group launched `reps` times (both groups are
+// idempotent: neither reads what it writes).
+int fused_step(const StepArgs &a, float *ws, hipStream_t s, hipEvent_t *ev, int reps = 1);
 size_t fused_workspace_bytes(int w, int h);
 int threads_for(int H);  // threads of the fused kernels for halo H
 

@@ -28,8 +28,10 @@
 // with P = vp and Q = tmp at entry, E = D^K if K is even else D^(K-1),
 // warm = Q[2] if K is even else P[2] (the pressure start), p^0 = warm:
 //   Q = (E.u, E.v, p^K, b)     P = (E - grad p^K, p^(K-1), b)
-// b = divergence of D^K.  Kernel D copies the pressure start to a workspace
-// plane so that kernel P's halo reads never race with its own writes.
+// b = divergence of D^K.  Kernel D only reads vp and tmp: it writes the
+// pressure start, b and E to workspace planes (16 B per cell), so that
+// kernel P's halo reads never race with its own writes, and kernel P writes
+// both outputs whole (32 B per cell) without reading them.
 
 #include <cmath>
 
@@ -95,7 +97,7 @@ size_t fused_workspace_bytes(int w, int h) {
     return scene_ws_floats(w, h, kMaxChunk + 1) * sizeof(float);
 }
 
-int fused_step(const StepArgs &s, float *ws, hipStream_t stream, hipEvent_t *ev) {
+int fused_step(const StepArgs &s, float *ws, hipStream_t stream, hipEvent_t *ev, int reps) {
     if (s.iters < 1) return set_error(PFS_ERR_INVALID, "fused step needs iters >= 1");
     Chunk plan[64];
     const int nchunks = chunk_plan(s.iters, plan, 64);
@@ -105,21 +107,24 @@ int fused_step(const StepArgs &s, float *ws, hipStream_t stream, hipEvent_t *ev)
     PressureArgs p = pressure_args(s.P, s.Q, s.image, s.itmp, s.w, s.h, s.dt, ws);
     const dim3 grid(d.ntiles);
 
+    if (!ev) reps = 1;
     if (ev) PFS_HIP_TRY(hipEventRecord(ev[0], stream));
-    for (int c = 0; c < nchunks; ++c) {
-        diffuse_chunk(d, c, plan[c], ws);
-        hipLaunchKernelGGL(diffuse_for(plan[c].H, plan[c].last), grid,
-                           dim3(threads_for(plan[c].H)), 0, stream, d);
-        PFS_TRY(check_launch("fused diffuse"));
-    }
+    for (int r = 0; r < reps; ++r)
+        for (int c = 0; c < nchunks; ++c) {
+            diffuse_chunk(d, c, plan[c], ws);
+            hipLaunchKernelGGL(diffuse_for(plan[c].H, plan[c].last), grid,
+                               dim3(threads_for(plan[c].H)), 0, stream, d);
+            PFS_TRY(check_launch("fused diffuse"));
+        }
     if (ev) PFS_HIP_TRY(hipEventRecord(ev[1], stream));
     if (ev) PFS_HIP_TRY(hipEventRecord(ev[2], stream));
-    for (int c = 0; c < nchunks; ++c) {
-        pressure_chunk(p, c, plan[c], ws, s.image, s.itmp);
-        hipLaunchKernelGGL(pressure_for(plan[c].H, plan[c].last), grid,
-                           dim3(threads_for(plan[c].H)), 0, stream, p);
-        PFS_TRY(check_launch("fused pressure"));
-    }
+    for (int r = 0; r < reps; ++r)
+        for (int c = 0; c < nchunks; ++c) {
+            pressure_chunk(p, c, plan[c], ws, s.image, s.itmp);
+            hipLaunchKernelGGL(pressure_for(plan[c].H, plan[c].last), grid,
+                               dim3(threads_for(plan[c].H)), 0, stream, p);
+            PFS_TRY(check_launch("fused pressure"));
+        }
     if (ev) PFS_HIP_TRY(hipEventRecord(ev[3], stream));
     return PFS_OK;
 }

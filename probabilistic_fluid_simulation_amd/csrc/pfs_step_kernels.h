@@ -38,9 +38,9 @@ __device__ __forceinline__ int wrap_index(int g, int n) {
 
 struct DiffuseArgs {
     const float *P;      // vp, [h][w][4]
-    float *Q;            // tmp, [h][w][4]
+    const float *Q;      // tmp, [h][w][4] (read only: its pressure, for even K)
     const float2 *dsrc;  // iterate plane read by a non-first chunk
-    float2 *ddst;        // iterate plane written by a non-last chunk
+    float2 *ddst;        // iterate plane written by a non-last chunk; E (last chunk)
     float *pw;           // pressure-start plane (last chunk)
     float *bpl;          // divergence plane (last chunk)
     int w, h, tiles_x, ntiles;
@@ -59,6 +59,7 @@ struct PressureArgs {
     float *pdst;        // iterate plane written by a non-last chunk
     const float *pw;
     const float *bpl;
+    const float2 *E;     // the diffused velocity kept for the output (kernel D's last chunk)
     const float *image;  // dye, [h][w][4], advected in the same pass; null: no dye
     float *itmp;
     int w, h, tiles_x, ntiles;
@@ -124,26 +125,22 @@ __device__ __forceinline__ void diffuse_tile(const DiffuseArgs &a, int tile) {
 
     const Strip<G> st(threadIdx.x, tile, a.tiles_x, a.w, a.h);
     const float2 *__restrict__ P2 = reinterpret_cast<const float2 *>(a.P);  // (u,v) at 2*cell
-    float4 *__restrict__ Q4 = reinterpret_cast<float4 *>(a.Q);
 
     // Every global load of the launch is issued here: the staged region, and
-    // (last chunk) the tile cells' tmp and pressure start, whose latency
-    // then hides behind the advection and the sweeps.
+    // (last chunk) the tile cells' pressure start, whose latency then hides
+    // behind the advection and the sweeps.
     float2 c[S];
 #pragma unroll
     for (int k = 0; k < S; ++k) {
         const int g = st.row_ok(k) ? st.gj[k] * a.w + st.gi : 0;
         c[k] = a.first ? P2[2 * g] : a.dsrc[g];
     }
-    float4 q[S];
     float warm[S];
     if (LAST) {
+        const float *__restrict__ W = a.k_even ? a.Q : a.P;
 #pragma unroll
-        for (int k = 0; k < S; ++k) {
-            const bool m = st.mine(k, a.w, a.h);
-            q[k] = m ? Q4[st.cell(k, a.w)] : make_float4(0.f, 0.f, 0.f, 0.f);
-            warm[k] = (m && !a.k_even) ? a.P[4 * st.cell(k, a.w) + 2] : q[k].z;
-        }
+        for (int k = 0; k < S; ++k)  // other rows load cell 0, unused
+            warm[k] = W[4 * (st.mine(k, a.w, a.h) ? st.cell(k, a.w) : 0) + 2];
     }
 
     float u[S], v[S];
@@ -190,9 +187,10 @@ __device__ __forceinline__ void diffuse_tile(const DiffuseArgs &a, int tile) {
     // out = (a*L + a*R + a*T + a*B + 1*C) / beta, left to right.  Sweep s
     // publishes alpha-scaled cells into plane (s+1)&1: sweep 1 uses plane 0
     // while plane 1 may still be read by the advection above.  The division
-    // takes the 5-FMA exactly rounded path; if any numerator of the strip
-    // is outside that path's range (zero, tiny, huge, Inf) the strip is redone
-    // with IEEE division.  Rows past RY are padding: computed, never used.
+    // takes an exactly rounded FMA path (3 or 5 operations); if any numerator
+    // of the strip is outside that path's range (zero, tiny, huge, Inf) the
+    // strip divides per element instead (IEEE division where needed).  Rows
+    // past RY are padding: computed, never used.
     float pu[S], pv[S];  // iterate before the last sweep (E for odd K)
     const bool fast = a.fast_div != 0;
     for (int s = 1; s <= NS; ++s) {
@@ -227,20 +225,23 @@ __device__ __forceinline__ void diffuse_tile(const DiffuseArgs &a, int tile) {
             lo = fminf(lo, fminf(fabsf(nu[k]), fabsf(nv[k])));
             hi = fmaxf(hi, fmaxf(fabsf(nu[k]), fabsf(nv[k])));
         }
-        if (short_div) {
+        // The range is tested before dividing, in the block that made the
+        // numerators (the compiler then keeps min3/max3 on them).
+        if (fast && lo >= 0x1p-96f && hi < 0x1p96f) {
+            if (short_div) {
 #pragma unroll
-            for (int k = 0; k < S; ++k) {
-                u[k] = div_fast3(nu[k], a.beta, a.rcp_beta);
-                v[k] = div_fast3(nv[k], a.beta, a.rcp_beta);
+                for (int k = 0; k < S; ++k) {
+                    u[k] = div_fast3(nu[k], a.beta, a.rcp_beta);
+                    v[k] = div_fast3(nv[k], a.beta, a.rcp_beta);
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < S; ++k) {
+                    u[k] = div_fast(nu[k], a.beta, a.rcp_beta);
+                    v[k] = div_fast(nv[k], a.beta, a.rcp_beta);
+                }
             }
         } else {
-#pragma unroll
-            for (int k = 0; k < S; ++k) {
-                u[k] = div_fast(nu[k], a.beta, a.rcp_beta);
-                v[k] = div_fast(nv[k], a.beta, a.rcp_beta);
-            }
-        }
-        if (!fast || !(lo >= 0x1p-96f) || !(hi < 0x1p96f)) {
 #pragma unroll
             for (int k = 0; k < S; ++k) {
                 u[k] = div_fixed(nu[k], a.beta, a.rcp_beta, fast);
@@ -273,7 +274,7 @@ __device__ __forceinline__ void diffuse_tile(const DiffuseArgs &a, int tile) {
         const int cell = st.cell(k, a.w);
         a.pw[cell] = warm[k];
         a.bpl[cell] = b;
-        Q4[cell] = make_float4(a.k_even ? u[k] : pu[k], a.k_even ? v[k] : pv[k], q[k].z, b);
+        a.ddst[cell] = make_float2(a.k_even ? u[k] : pu[k], a.k_even ? v[k] : pv[k]);
     }
 }
 
@@ -299,16 +300,16 @@ __device__ __forceinline__ void pressure_tile(const PressureArgs &a, int tile) {
             rhs[k] = a.bpl[g];
         }
     }
-    float4 q[S];  // (E.u, E.v, start, b) of the tile cells
+    float2 e[S];  // E of the tile cells (other rows load cell 0, unused)
     if (LAST) {
 #pragma unroll
-        for (int k = 0; k < S; ++k)
-            q[k] = st.mine(k, a.w, a.h) ? Q4[st.cell(k, a.w)] : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int k = 0; k < S; ++k) e[k] = a.E[st.mine(k, a.w, a.h) ? st.cell(k, a.w) : 0];
     }
 
     // Pressure Jacobi sweeps (src/fluid.cpp:240-257):
     // p' = (pL + pR + pT + pB + 1*b) / 4, left to right.
     float pp[S];  // p^(K-1)
+#pragma unroll 1
     for (int s = 1; s <= NS; ++s) {
         float *B = plane + (s & 1) * PL;
 #pragma unroll
@@ -343,35 +344,28 @@ __device__ __forceinline__ void pressure_tile(const PressureArgs &a, int tile) {
     for (int k = 0; k < S; ++k)
         if (st.active) B[(st.r0 + k) * RX + st.x] = p[k];
     __syncthreads();
-    float un[S], vn[S];
+    // The new velocities also go to LDS as they are made, for the dye pass.
+    __shared__ float2 Vn[LAST ? G::TX * G::TY : 1];
+    const bool dye = a.image != nullptr;
 #pragma unroll
     for (int k = 0; k < S; ++k) {
-        un[k] = vn[k] = 0.0f;
         if (!st.mine(k, a.w, a.h)) continue;
         const int o = (st.r0 + k) * RX + st.x;
         const float gradX = (B[o + 1] - B[o - 1]) * a.dt / 2.0f;
         const float gradY = (B[o + RX] - B[o - RX]) * a.dt / 2.0f;
         const int cell = st.cell(k, a.w);
-        un[k] = q[k].x - gradX;
-        vn[k] = q[k].y - gradY;
-        P4[cell] = make_float4(un[k], vn[k], pp[k], rhs[k]);
-        Q4[cell] = make_float4(q[k].x, q[k].y, p[k], rhs[k]);
+        const float un = e[k].x - gradX, vn = e[k].y - gradY;
+        P4[cell] = make_float4(un, vn, pp[k], rhs[k]);
+        Q4[cell] = make_float4(e[k].x, e[k].y, p[k], rhs[k]);
+        if (dye) Vn[(st.r0 + k - H) * G::TX + (st.x - H)] = make_float2(un, vn);
     }
-    if (a.image == nullptr) return;
+    if (!dye) return;
 
     // Dye advection of the tile's pixels (src/fluid.cpp:89-124 with the
     // image the size of the field, so pixel (i, j) samples cell (i, j)),
     // pixels spread over all threads so that their gathers are in flight
-    // together.  The new velocities are handed over through LDS: both
-    // planes (>= TX*TY float2) are free once every thread is past the
-    // gradient.
-    static_assert(2 * PL >= 2 * G::TX * G::TY, "velocity hand-over does not fit");
-    __syncthreads();
-    float2 *Vn = reinterpret_cast<float2 *>(plane);
-#pragma unroll
-    for (int k = 0; k < S; ++k)
-        if (st.mine(k, a.w, a.h))
-            Vn[(st.r0 + k - H) * G::TX + (st.x - H)] = make_float2(un[k], vn[k]);
+    // together.  (Staging the dye tile in LDS during the sweeps, by the LDS
+    // DMA path, measured slower: its burst competes with the sweeps.)
     __syncthreads();
     const float4 *__restrict__ I4 = reinterpret_cast<const float4 *>(a.image);
     float4 *__restrict__ O4 = reinterpret_cast<float4 *>(a.itmp);
@@ -398,11 +392,13 @@ __device__ __forceinline__ void pressure_tile(const PressureArgs &a, int tile) {
 
 // ---- host-side helpers shared by the single-scene and batched launchers ----
 
-// Workspace of one scene (floats, N = w*h): pw [0, N), b [N, 2N), and for
-// K > kMaxChunk the iterate planes d0 [2N, 4N), d1 [4N, 6N) (float2),
-// p0 [6N, 7N), p1 [7N, 8N).
+// Workspace of one scene (floats, N = w*h): pw [0, N), b [N, 2N), the
+// float2 planes d0 [2N, 4N), d1 [4N, 6N) and, for K > kMaxChunk, the
+// pressure iterate planes p0 [6N, 7N), p1 [7N, 8N).  Chunk c of the
+// diffusion writes d[c & 1]: the next chunk's input, or E when c is the
+// last chunk (one chunk: d0 only).
 inline size_t scene_ws_floats(int w, int h, int iters) {
-    return (iters > kMaxChunk ? 8 : 2) * (size_t)w * (size_t)h;
+    return (iters > kMaxChunk ? 8 : 4) * (size_t)w * (size_t)h;
 }
 
 // The launch sequence of one step: chunk c runs ns sweeps with halo H.
@@ -424,7 +420,7 @@ inline int chunk_plan(int iters, Chunk *out, int max_chunks) {
 }
 
 // Scene-wide arguments of the two kernels (chunk fields set per launch).
-inline DiffuseArgs diffuse_args(const float *P, float *Q, int w, int h, float dt, float visc,
+inline DiffuseArgs diffuse_args(const float *P, const float *Q, int w, int h, float dt, float visc,
                                 int iters, float *ws) {
     DiffuseArgs d{};
     const size_t N = (size_t)w * h;
@@ -501,6 +497,7 @@ inline void pressure_chunk(PressureArgs &p, int c, const Chunk &ch, float *ws,
                            const float *image, float *itmp) {
     const size_t N = (size_t)p.w * p.h;
     float *ppl[2] = {ws + 6 * N, ws + 7 * N};
+    p.E = reinterpret_cast<const float2 *>(ws + 2 * N + 2 * N * (c & 1));
     p.first = ch.first;
     p.psrc = ppl[(c + 1) & 1];
     p.pdst = ppl[c & 1];

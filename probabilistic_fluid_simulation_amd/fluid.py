@@ -225,10 +225,13 @@ def get_step_path() -> str:
 
 
 def profile_step(vp: VPField, tmp: VPField, dt: float, viscosity: float, iters: int,
-                 image: Optional[VPField] = None, itmp: Optional[VPField] = None, stream=None):
+                 image: Optional[VPField] = None, itmp: Optional[VPField] = None, stream=None,
+                 reps: int = 1):
     """One fused step (or, with image/itmp, one fluid_timestep) timed per
-    kernel group with HIP events on the launch stream: returns
-    (advect+diffusion+divergence ms, pressure+projection(+dye) ms)."""
+    kernel group with HIP events on the launch stream, each group launched
+    `reps` times back to back (the buffers still advance by one step):
+    returns (advect+diffusion+divergence ms, pressure+projection(+dye) ms)
+    per launch."""
     ms = (ctypes.c_float * 2)()
     if image is None:
         b = _Bound(vp, tmp)
@@ -236,7 +239,7 @@ def profile_step(vp: VPField, tmp: VPField, dt: float, viscosity: float, iters: 
     else:
         b = _Bound(vp, tmp, image, itmp)
         args = (b.ptr(0), b.ptr(1), b.ptr(2), b.ptr(3))
-    _lib.call("pfs_profile_step", *args, dt, viscosity, iters, _stream(stream), ms, 2)
+    _lib.call("pfs_profile_step", *args, dt, viscosity, iters, reps, _stream(stream), ms, 2)
     b.sync_back()
     return float(ms[0]), float(ms[1])
 

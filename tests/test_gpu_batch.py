@@ -108,7 +108,7 @@ def test_batch_workspace_and_errors():
     s = _scene(0, (64, 64, 64, 64, 8, 4))
     vp, tmp, im, itmp = to_dev(s)
     b.add(vp, tmp, im, itmp, s.dt, s.visc, 8)
-    assert b.workspace_bytes == 2 * 4 * 64 * 64
+    assert b.workspace_bytes == 4 * 4 * 64 * 64  # pw, b and the float2 E plane
     with pytest.raises(Exception):
         b.add(vp, vp, im, itmp, s.dt, s.visc, 8)  # aliasing
     with pytest.raises(Exception):

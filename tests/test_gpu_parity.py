@@ -221,6 +221,17 @@ def test_profile_step_times_and_advances(orc):
     assert_bitexact([f.numpy() for f in (vp, tmp, im, itmp)], [a.vp, a.tmp, a.image, a.itmp])
 
 
+@pytest.mark.parametrize("iters", [8, 12])
+def test_profile_step_repeated_groups_advance_one_step(orc, iters):
+    """Each kernel group launched several times back to back (idempotent
+    groups, incl. K > 8 chunks) still advances the buffers by one step."""
+    s = scenes.random_scene(130, 70, 130, 70, seed=15, dt=0.6, visc=0.15, iters=iters)
+    vp, tmp, im, itmp = to_dev(s)
+    ms = fluid.profile_step(vp, tmp, s.dt, s.visc, s.iters, image=im, itmp=itmp, reps=4)
+    assert all(m > 0 for m in ms)
+    assert_bitexact([f.numpy() for f in (vp, tmp, im, itmp)], oracle_run(orc, s, 1))
+
+
 # ------------------------------------------------------------------- cell keys
 def _keys_numpy(vp, dt):
     """Back-trace keys of src/fluid.cpp:39-57 evaluated with numpy float32
