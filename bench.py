@@ -53,27 +53,66 @@ def pmc_traffic(kernel: str):
 
 
 def cpu_baseline(scene, budget_steps: int) -> dict:
+    """The reference's CPU solver on this host's cores, on bounded samples
+    of the workload (each a number of its first steps):
+    - headline: the reference's own src/fluid.cpp built -O2 -ffp-contract=off,
+      one thread (fluid.cpp has no threading); the C restatement stands in
+      where the reference could not be built;
+    - "openmp": the C restatement, row loops parallel over the OpenMP
+      threads this process gets (OMP_NUM_THREADS, else every core), 4x the
+      sample;
+    - "reference_O0": fluid.cpp at -O0, the reference CMake build's default,
+      1/8 of the sample.
+    All three are bit-identical to each other and to the GPU."""
     import oracle
 
-    s = scene.copy()
-    ref = oracle.ref(s.iters)
+    def timed(run, steps):
+        s = scene.copy()
+        t0 = time.perf_counter()
+        run(s, steps)
+        return time.perf_counter() - t0
+
+    ref = oracle.ref(scene.iters)
+    orc = oracle.get(32)
     if ref is not None:
-        kind, runner = "reference", lambda: ref.run(s.vp, s.tmp, s.image, s.itmp, s.dt, s.visc,
-                                                     budget_steps)
-        what = "reference src/fluid.cpp (g++ -O2 -ffp-contract=off), 1 thread"
+        kind, what = "reference", "reference src/fluid.cpp (g++ -O2 -ffp-contract=off), 1 thread"
+        run = lambda s, n: ref.run(s.vp, s.tmp, s.image, s.itmp, s.dt, s.visc, n)
     else:
-        orc = oracle.get(32)
-        orc.set_threads(1)
-        kind, runner = "port", lambda: orc.run(s.vp, s.tmp, s.image, s.itmp, s.dt, s.visc,
-                                               s.iters, budget_steps)
-        what = "C restatement of src/fluid.cpp (gcc -O2 -ffp-contract=off), 1 thread"
-    t0 = time.perf_counter()
-    runner()
-    dt = time.perf_counter() - t0
-    return {"value": scene.cells * budget_steps / dt, "unit": "particle-steps/s", "cores": 1,
-            "kind": kind,
-            "sample": f"{scene.name}: first {budget_steps} of {scene.steps} steps, {what}, "
-                      f"{dt:.1f} s"}
+        kind, what = "port", "C restatement of src/fluid.cpp (gcc -O2 -ffp-contract=off), 1 thread"
+
+        def run(s, n):
+            orc.set_threads(1)
+            orc.run(s.vp, s.tmp, s.image, s.itmp, s.dt, s.visc, s.iters, n)
+    dt = timed(run, budget_steps)
+    out = {"value": scene.cells * budget_steps / dt, "unit": "particle-steps/s", "cores": 1,
+           "kind": kind,
+           "sample": f"{scene.name}: first {budget_steps} of {scene.steps} steps, {what}, "
+                     f"{dt:.1f} s",
+           "nproc": os.cpu_count()}
+
+    threads = orc.max_threads()
+    omp_steps = 4 * budget_steps
+
+    def run_omp(s, n):
+        orc.set_threads(threads)
+        orc.run(s.vp, s.tmp, s.image, s.itmp, s.dt, s.visc, s.iters, n)
+    dt = timed(run_omp, omp_steps)
+    orc.set_threads(threads)
+    out["openmp"] = {"value": scene.cells * omp_steps / dt, "unit": "particle-steps/s",
+                     "cores": threads, "kind": "port",
+                     "sample": f"{scene.name}: first {omp_steps} steps, C restatement of "
+                               f"src/fluid.cpp (gcc -O2 -ffp-contract=off -fopenmp), row loops "
+                               f"over {threads} threads, {dt:.1f} s"}
+    ref0 = oracle.ref(scene.iters, "O0")
+    if ref0 is not None:
+        o0_steps = max(1, budget_steps // 8)
+        dt = timed(lambda s, n: ref0.run(s.vp, s.tmp, s.image, s.itmp, s.dt, s.visc, n),
+                   o0_steps)
+        out["reference_O0"] = {"value": scene.cells * o0_steps / dt, "unit": "particle-steps/s",
+                               "cores": 1, "kind": "reference",
+                               "sample": f"{scene.name}: first {o0_steps} steps, reference "
+                                         f"src/fluid.cpp (g++ -O0), 1 thread, {dt:.1f} s"}
+    return out
 
 
 def batched_leg(n_scenes: int, steps: int) -> dict:

@@ -147,10 +147,12 @@ class Oracle:
 
 
 class RefFluid:
-    """The reference's own src/fluid.cpp, built for one Jacobi count K."""
+    """The reference's own src/fluid.cpp, built for one Jacobi count K
+    (opt "O0": the -O0 build, made for K = 8 only)."""
 
-    def __init__(self, iters: int):
-        path = os.path.join(REF_DIR, f"libfluidref_k{iters}.so")
+    def __init__(self, iters: int, opt: str = "O2"):
+        suffix = "" if opt == "O2" else f"_{opt}"
+        path = os.path.join(REF_DIR, f"libfluidref_k{iters}{suffix}.so")
         if not os.path.exists(path):
             raise FileNotFoundError(path)
         self.lib = ctypes.CDLL(path)
@@ -200,8 +202,9 @@ class RefFluid:
         return self.b.back(f, (vp, tmp, image, itmp))
 
 
-def ref_available(iters: int) -> bool:
-    return os.path.exists(os.path.join(REF_DIR, f"libfluidref_k{iters}.so"))
+def ref_available(iters: int, opt: str = "O2") -> bool:
+    suffix = "" if opt == "O2" else f"_{opt}"
+    return os.path.exists(os.path.join(REF_DIR, f"libfluidref_k{iters}{suffix}.so"))
 
 
 _ORACLES = {}
@@ -213,5 +216,5 @@ def get(prec: int = 32) -> Oracle:
     return _ORACLES[prec]
 
 
-def ref(iters: int) -> Optional[RefFluid]:
-    return RefFluid(iters) if ref_available(iters) else None
+def ref(iters: int, opt: str = "O2") -> Optional[RefFluid]:
+    return RefFluid(iters, opt) if ref_available(iters, opt) else None

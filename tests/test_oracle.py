@@ -127,6 +127,19 @@ def test_against_compiled_reference(orc, k):
         assert all(same(x, y) for x, y in zip(ra, rb))
 
 
+def test_reference_O0_build_equals_O2_build():
+    """The reference CMake build's default -O0 and the -O2 build used as the
+    CPU baseline give the same bits (no FMA contraction, same libm calls)."""
+    r0, r2 = oracle.ref(8, "O0"), oracle.ref(8)
+    if r0 is None or r2 is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    s = scenes.random_scene(40, 24, 33, 19, seed=7, dt=0.8, visc=0.25, iters=8)
+    a, b = s.copy(), s.copy()
+    ra = r0.run(a.vp, a.tmp, a.image, a.itmp, s.dt, s.visc, 3)
+    rb = r2.run(b.vp, b.tmp, b.image, b.itmp, s.dt, s.visc, 3)
+    assert all(same(x, y) for x, y in zip(ra, rb))
+
+
 # ------------------------------------------------------------------ closed forms
 def _field(w, h, u=None, v=None, p=None, b=None, dtype=np.float32):
     f = np.zeros((h, w, 4), dtype=dtype)
