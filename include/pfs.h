@@ -216,6 +216,34 @@ int pfs_simulate_fluid_step_f64(double *vp, double *tmp, int vx, int vy, double 
                                 double viscosity, int iters_diffuse, int iters_pressure,
                                 void *workspace, size_t workspace_bytes, void *stream);
 
+/* ---- PNG files of the reference's harness ---------------------------------
+ * The reference reads its input image and velocity field with libpng's
+ * simplified API as 8-bit RGBA and converts each byte to float (byte/255.0),
+ * and writes every frame as 8-bit RGBA with (png_byte)(x*255.0)
+ * (includes/utils.hpp:32-104, 115-150; src/main.cpp:126-140, 59-67).  These
+ * entry points do the same on the host without libpng: any PNG colour type,
+ * bit depth and interlace, converted to RGBA8 as libpng 1.6's simplified
+ * API does (gamma to sRGB included); floats are [h][w][4]. */
+int pfs_png_size(const char *path, int *width, int *height);
+/* out: width*height*4 bytes. */
+int pfs_png_read_rgba8(const char *path, uint8_t *out, size_t n_bytes);
+/* out: width*height*4 floats, each (float)((double)byte / 255.0) (utils.hpp:83-85). */
+int pfs_png_read(const char *path, float *out, size_t n_floats);
+int pfs_png_write_rgba8(const char *path, const uint8_t *rgba, int width, int height);
+/* data: width*height*4 floats; byte = (png_byte)(x * 255.0) as the reference
+ * evaluates it on x86-64 (utils.hpp:129): truncation, low byte of the 32-bit
+ * integer for values outside [0, 256). */
+int pfs_png_write(const char *path, const float *data, int width, int height);
+
+/* ---- the simulation driver -------------------------------------------------
+ * src/main.cpp:71-280 on the GPU: argv as the reference's
+ *   <n_timesteps> <delta_t> <viscosity> <input_image> <velocity_field> [output_dir]
+ * plus options (--iters K, --every N, --npy, --dump DIR, --per-op,
+ * --buffers N, --quiet); the same messages; frames copied out on a second
+ * stream through pinned buffers while the steps run on.  Returns the
+ * process exit status.  The pfs_sim executable is main() around it. */
+int pfs_sim_main(int argc, char **argv);
+
 /* ---- debug: the integer "neighbour lists" ------------------------------
  * Back-trace cell keys of advect (src/fluid.cpp:55-57): keys[(j*w+i)*4 +
  * {0,1,2,3}] = {i0, j0, i1, j1} for every cell.  `keys` is a device int32

@@ -86,6 +86,12 @@ SIGNATURES = {
                                         _V, ctypes.POINTER(PreciseReport)],
     "pfs_simulate_fluid_step_f64": [_V, _V, _I, _I, ctypes.c_double, ctypes.c_double, _I, _I, _V,
                                     ctypes.c_size_t, _V],
+    "pfs_png_size": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
+    "pfs_png_read_rgba8": [ctypes.c_char_p, _V, ctypes.c_size_t],
+    "pfs_png_read": [ctypes.c_char_p, _V, ctypes.c_size_t],
+    "pfs_png_write_rgba8": [ctypes.c_char_p, _V, _I, _I],
+    "pfs_png_write": [ctypes.c_char_p, _V, _I, _I],
+    "pfs_sim_main": [_I, ctypes.POINTER(ctypes.c_char_p)],
     "pfs_version": [],
     "pfs_last_error": [],
 }

@@ -1,0 +1,113 @@
+"""The simulation driver (pfs_sim_main, src/main.cpp:71-280) on the GPU, run
+in process: every written frame and the final state against the oracle
+stepping the same inputs (src/main.cpp:219-240), bit for bit."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from probabilistic_fluid_simulation_amd import pngio  # noqa: E402
+from probabilistic_fluid_simulation_amd.sim import sim_main  # noqa: E402
+
+import png_util  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _inputs(tmp_path, fw, fh, iw, ih, seed=0):
+    """A 16-bit RGBA velocity field (like the reference's inputs/velocity_fields)
+    and an 8-bit RGBA image."""
+    rng = np.random.default_rng(seed)
+    vel = rng.integers(0, 65536, size=(fh, fw, 4), dtype=np.int64)
+    vel[..., 3] = 65535
+    img = rng.integers(0, 256, size=(ih, iw, 4), dtype=np.int64)
+    pv = png_util.write(str(tmp_path / "vel.png"), vel, 16, 6)
+    pi = png_util.write(str(tmp_path / "img.png"), img, 8, 6)
+    return pi, pv
+
+
+def _oracle_frames(orc, scene, steps):
+    """Images after each step and the final (vp, tmp, image, itmp)."""
+    s = scene
+    vp, tmp, image, itmp = s.vp.copy(), s.tmp.copy(), s.image.copy(), s.itmp.copy()
+    frames = []
+    for _ in range(steps):
+        vp, tmp = orc.simulate_fluid_step(vp, tmp, s.dt, s.visc, s.iters)
+        image, itmp = orc.advect_color_step(image, itmp, vp, s.dt)
+        frames.append(image.copy())
+    return frames, (vp, tmp, image, itmp)
+
+
+def _load_dump(d):
+    return [np.load(os.path.join(d, f"{n}.npy"), allow_pickle=False)
+            for n in ("vp", "tmp", "image", "itmp")]
+
+
+@pytest.mark.parametrize("dims", [(40, 24, 33, 19), (48, 32, 48, 32)],
+                         ids=["dye-other-size", "dye-fused"])
+def test_frames_and_final_state_match_oracle(orc, tmp_path, dims):
+    pi, pv = _inputs(tmp_path, *dims)
+    out, dump = tmp_path / "frames", tmp_path / "dump"
+    out.mkdir()
+    dump.mkdir()
+    steps, dt, visc, k = 5, 0.5, 0.1, 4
+    rc = sim_main([str(steps), str(dt), str(visc), pi, pv, str(out), "--iters", str(k),
+                   "--dump", str(dump), "--buffers", "2"])
+    assert rc == 0
+    scene = pngio.load_scene(pi, pv, np.float32(dt), np.float32(visc), iters=k)
+    frames, final = _oracle_frames(orc, scene, steps)
+    for i, f in enumerate(frames):
+        got = pngio.read_png_rgba8(str(out / f"{i}.png"))
+        np.testing.assert_array_equal(got, pngio.bytes_from_floats(f), err_msg=f"frame {i}")
+    for name, g, w in zip(("vp", "tmp", "image", "itmp"), _load_dump(str(dump)), final):
+        assert g.tobytes() == w.tobytes(), name
+
+
+def test_npy_frames_every_and_per_op(orc, tmp_path):
+    pi, pv = _inputs(tmp_path, 30, 20, 30, 20, seed=1)
+    steps, k = 6, 3
+    scene = pngio.load_scene(pi, pv, np.float32(0.75), np.float32(0.2), iters=k)
+    frames, final = _oracle_frames(orc, scene, steps)
+    for per_op in (False, True):
+        out = tmp_path / f"npy{int(per_op)}"
+        dump = tmp_path / f"dump{int(per_op)}"
+        out.mkdir()
+        dump.mkdir()
+        args = [str(steps), "0.75", "0.2", pi, pv, str(out), "--iters", str(k), "--npy",
+                "--every", "2", "--dump", str(dump), "--quiet"]
+        if per_op:
+            args.append("--per-op")
+        assert sim_main(args) == 0
+        assert sorted(os.listdir(out)) == ["0.npy", "2.npy", "4.npy"]
+        for i in (0, 2, 4):
+            got = np.load(str(out / f"{i}.npy"), allow_pickle=False)
+            assert got.tobytes() == frames[i].tobytes(), (per_op, i)
+        for g, w in zip(_load_dump(str(dump)), final):
+            assert g.tobytes() == w.tobytes()
+
+
+def test_timing_mode_and_npy_inputs(orc, tmp_path):
+    """Five arguments: no frames (src/main.cpp:108-111).  .npy inputs are the
+    raw float fields (the velocity already rescaled)."""
+    scene = pngio.load_scene(*_inputs(tmp_path, 17, 13, 17, 13, seed=2), np.float32(0.5),
+                             np.float32(0.1), iters=2)
+    pi, pv = str(tmp_path / "i.npy"), str(tmp_path / "v.npy")
+    np.save(pi, scene.image)
+    np.save(pv, scene.vp)
+    assert sim_main(["3", "0.5", "0.1", pi, pv, "--iters", "2"]) == 0
+    dump = tmp_path / "d"
+    dump.mkdir()
+    assert sim_main(["3", "0.5", "0.1", pi, pv, "--iters", "2", "--dump", str(dump)]) == 0
+    _, final = _oracle_frames(orc, scene, 3)
+    for g, w in zip(_load_dump(str(dump)), final):
+        assert g.tobytes() == w.tobytes()
+
+
+def test_unwritable_output_dir_fails(tmp_path):
+    pi, pv = _inputs(tmp_path, 16, 16, 16, 16, seed=3)
+    assert sim_main(["2", "0.5", "0.1", pi, pv, str(tmp_path / "no" / "such"), "--iters", "2",
+                     "--quiet"]) == 1
