@@ -216,6 +216,53 @@ int pfs_simulate_fluid_step_f64(double *vp, double *tmp, int vx, int vy, double 
                                 double viscosity, int iters_diffuse, int iters_pressure,
                                 void *workspace, size_t workspace_bytes, void *stream);
 
+/* ---- particle layer ----------------------------------------------------------
+ * No counterpart in the reference, which has no particles.  Tracers (SoA
+ * float x[n], y[n] in field-cell units, periodic [0, W) x [0, H)) carried by
+ * the grid's velocity, and a uniform neighbour grid over them: cell keys, a
+ * stable radix sort, cell start/end tables, neighbour lists and an SPH
+ * density.  All integer outputs are deterministic (a particle's neighbours
+ * are listed by 3x3 cell rows, then columns, then sorted position). */
+typedef struct pfs_ngrid {
+    int n;                    /* particles */
+    float domain_w, domain_h; /* periodic domain, same units as x, y */
+    float radius;             /* neighbour radius r */
+    int gx, gy;               /* cells per axis (>= 3, each at least r wide) */
+    float kx, ky;             /* gx / domain_w, gy / domain_h: cell = (int)(x * kx) */
+} pfs_ngrid;
+
+/* One forward step: x += (dt * u(x, y)) / W, y += (dt * v(x, y)) / H, with u, v
+ * bilinear in vp (channels 0, 1) at the position, corner keys and blend as
+ * src/fluid.cpp:55-66; positions wrap into [0, W) x [0, H). */
+int pfs_particles_advect(float *x, float *y, int n, const pfs_field *vp, float dt, void *stream);
+/* Fills *g: gx = floor(W / r) (lowered until cells are >= r wide), likewise gy. */
+int pfs_ngrid_init(pfs_ngrid *g, int n, float domain_w, float domain_h, float radius);
+size_t pfs_ngrid_workspace_bytes(const pfs_ngrid *g);
+/* keys[n]: cell of particle i; order[n]: particle ids sorted by (key, id);
+ * cell_start/cell_end[gx*gy]: [start, end) of each cell in sorted order
+ * (0, 0 when empty); xs, ys[n]: positions in sorted order.  The workspace
+ * keeps the sorted keys for pfs_ngrid_neighbors / pfs_ngrid_density. */
+int pfs_ngrid_build(const pfs_ngrid *g, const float *x, const float *y, uint32_t *keys,
+                    uint32_t *order, uint32_t *cell_start, uint32_t *cell_end, float *xs, float *ys,
+                    void *workspace, size_t workspace_bytes, void *stream);
+/* counts[i]: neighbours of particle i within r (periodic distance, d^2 < r^2,
+ * itself excluded); offsets[i]: their exclusive scan; *total (device): the
+ * sum.  With nbr non-NULL also the lists, nbr[offsets[i] + k] (particle
+ * ids); this waits for *total to check nbr_capacity. */
+int pfs_ngrid_neighbors(const pfs_ngrid *g, const float *xs, const float *ys, const uint32_t *order,
+                        const uint32_t *cell_start, const uint32_t *cell_end, uint32_t *counts,
+                        uint32_t *offsets, uint32_t *nbr, size_t nbr_capacity, uint32_t *total,
+                        void *workspace, size_t workspace_bytes, void *stream);
+/* rho[i] = sum over i and its neighbours of mass * 4/(pi r^8) * (r^2 - d^2)^3
+ * (2D poly6), Kahan-compensated FP32 in the neighbour order, self first. */
+int pfs_ngrid_density(const pfs_ngrid *g, const float *xs, const float *ys, const uint32_t *order,
+                      const uint32_t *cell_start, const uint32_t *cell_end, float mass, float *rho,
+                      void *workspace, size_t workspace_bytes, void *stream);
+/* Exclusive prefix sum of n uint32 (wavefront DPP scans); *total (device,
+ * may be NULL) gets the sum.  Workspace: (ceil(n / 4096) + 1) * 4 bytes. */
+int pfs_scan_exclusive_u32(const uint32_t *in, uint32_t *out, int n, uint32_t *total,
+                           void *workspace, size_t workspace_bytes, void *stream);
+
 /* ---- PNG files of the reference's harness ---------------------------------
  * The reference reads its input image and velocity field with libpng's
  * simplified API as 8-bit RGBA and converts each byte to float (byte/255.0),

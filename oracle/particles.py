@@ -1,0 +1,83 @@
+"""ctypes binding of oracle/pfs_particles.c, the CPU restatement of the
+particle layer -- TEST INFRASTRUCTURE ONLY (see that file's header)."""
+from __future__ import annotations
+
+import ctypes
+from typing import Tuple
+
+import numpy as np
+
+from . import ORACLE_SO
+
+_lib = None
+
+
+def _load() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        lib = ctypes.CDLL(ORACLE_SO)
+        P, I, F = ctypes.c_void_p, ctypes.c_int, ctypes.c_float
+        lib.orcp_advect.argtypes = [P, P, I, P, I, I, I, F]
+        lib.orcp_advect.restype = None
+        lib.orcp_ngrid_build.argtypes = [P, P, I, F, F, I, I, P, P, P, P, P, P]
+        lib.orcp_ngrid_build.restype = I
+        lib.orcp_ngrid_neighbors.argtypes = [P, P, P, I, P, P, P, I, I, F, F, F, P, P, P]
+        lib.orcp_ngrid_neighbors.restype = ctypes.c_longlong
+        lib.orcp_ngrid_density.argtypes = [P, P, P, I, P, P, P, I, I, F, F, F, F, P]
+        lib.orcp_ngrid_density.restype = None
+        _lib = lib
+    return _lib
+
+
+def _p(a: np.ndarray) -> ctypes.c_void_p:
+    assert a.flags.c_contiguous
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def advect(x: np.ndarray, y: np.ndarray, vp: np.ndarray, dt: float) -> None:
+    """In place, float32 x, y; vp (H, W, C) float32."""
+    h, w, c = vp.shape
+    vp = np.ascontiguousarray(vp, dtype=np.float32)
+    _load().orcp_advect(_p(x), _p(y), len(x), _p(vp), w, h, c, dt)
+
+
+class Grid:
+    """The neighbour grid of n particles (parameters as pfs_ngrid)."""
+
+    def __init__(self, x, y, W, H, r, gx, gy, kx, ky):
+        self.x = np.ascontiguousarray(x, dtype=np.float32)
+        self.y = np.ascontiguousarray(y, dtype=np.float32)
+        self.n = len(self.x)
+        self.W, self.H, self.r, self.gx, self.gy = W, H, r, gx, gy
+        nc = gx * gy
+        self.keys = np.empty(self.n, np.uint32)
+        self.order = np.empty(self.n, np.uint32)
+        self.start = np.empty(nc, np.uint32)
+        self.end = np.empty(nc, np.uint32)
+        self.xs = np.empty(self.n, np.float32)
+        self.ys = np.empty(self.n, np.float32)
+        rc = _load().orcp_ngrid_build(_p(self.x), _p(self.y), self.n, kx, ky, gx, gy,
+                                      _p(self.keys), _p(self.order), _p(self.start),
+                                      _p(self.end), _p(self.xs), _p(self.ys))
+        assert rc == 0
+        self.keys_sorted = np.ascontiguousarray(self.keys[self.order])
+
+    def neighbors(self) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """(counts by id, exclusive offsets, lists)."""
+        lib = _load()
+        counts = np.empty(self.n, np.uint32)
+        args = (_p(self.xs), _p(self.ys), _p(self.order), self.n, _p(self.keys_sorted),
+                _p(self.start), _p(self.end), self.gx, self.gy, self.W, self.H, self.r)
+        total = lib.orcp_ngrid_neighbors(*args, _p(counts), None, None)
+        offsets = np.zeros(self.n, np.uint32)
+        offsets[1:] = np.cumsum(counts, dtype=np.uint64)[:-1].astype(np.uint32)
+        nbr = np.empty(max(int(total), 1), np.uint32)
+        lib.orcp_ngrid_neighbors(*args, _p(counts), _p(offsets), _p(nbr))
+        return counts, offsets, nbr[:int(total)]
+
+    def density(self, mass: float) -> np.ndarray:
+        rho = np.empty(self.n, np.float32)
+        _load().orcp_ngrid_density(_p(self.xs), _p(self.ys), _p(self.order), self.n,
+                                   _p(self.keys_sorted), _p(self.start), _p(self.end), self.gx,
+                                   self.gy, self.W, self.H, self.r, mass, _p(rho))
+        return rho

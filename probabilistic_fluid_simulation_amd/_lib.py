@@ -45,7 +45,15 @@ class PfsError(RuntimeError):
         self.code = code
 
 
+class PfsNGrid(ctypes.Structure):
+    """pfs_ngrid (include/pfs.h): the particle layer's neighbour grid."""
+    _fields_ = [("n", ctypes.c_int), ("domain_w", ctypes.c_float), ("domain_h", ctypes.c_float),
+                ("radius", ctypes.c_float), ("gx", ctypes.c_int), ("gy", ctypes.c_int),
+                ("kx", ctypes.c_float), ("ky", ctypes.c_float)]
+
+
 _P = ctypes.POINTER(PfsField)
+_G = ctypes.POINTER(PfsNGrid)
 _V = ctypes.c_void_p
 _F = ctypes.c_float
 _I = ctypes.c_int
@@ -92,13 +100,22 @@ SIGNATURES = {
     "pfs_png_write_rgba8": [ctypes.c_char_p, _V, _I, _I],
     "pfs_png_write": [ctypes.c_char_p, _V, _I, _I],
     "pfs_sim_main": [_I, ctypes.POINTER(ctypes.c_char_p)],
+    "pfs_particles_advect": [_V, _V, _I, _P, _F, _V],
+    "pfs_ngrid_init": [_G, _I, _F, _F, _F],
+    "pfs_ngrid_workspace_bytes": [_G],
+    "pfs_ngrid_build": [_G, _V, _V, _V, _V, _V, _V, _V, _V, _V, ctypes.c_size_t, _V],
+    "pfs_ngrid_neighbors": [_G, _V, _V, _V, _V, _V, _V, _V, _V, ctypes.c_size_t, _V, _V,
+                            ctypes.c_size_t, _V],
+    "pfs_ngrid_density": [_G, _V, _V, _V, _V, _V, _F, _V, _V, ctypes.c_size_t, _V],
+    "pfs_scan_exclusive_u32": [_V, _V, _I, _V, _V, ctypes.c_size_t, _V],
     "pfs_version": [],
     "pfs_last_error": [],
 }
 _RESTYPES = {"pfs_version": ctypes.c_char_p, "pfs_last_error": ctypes.c_char_p,
              "pfs_step_workspace_bytes": ctypes.c_size_t,
              "pfs_batch_workspace_bytes": ctypes.c_size_t, "pfs_batch_stream": ctypes.c_void_p,
-             "pfs_precise_workspace_bytes": ctypes.c_size_t}
+             "pfs_precise_workspace_bytes": ctypes.c_size_t,
+             "pfs_ngrid_workspace_bytes": ctypes.c_size_t}
 
 _lock = threading.Lock()
 _lib = None

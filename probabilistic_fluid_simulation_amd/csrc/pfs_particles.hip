@@ -1,0 +1,570 @@
+// Particle layer over the grid fluid (gfx950).  No counterpart in the
+// reference, which has no particles: these are tracers carried by the grid's
+// velocity field, with a uniform neighbour grid over them.
+//
+//   pfs_particles_advect     x += (dt*u(x))/W, y likewise (the reference's
+//                            back-trace, src/fluid.cpp:39-41, run forward),
+//                            u bilinear at the position with the reference's
+//                            corner keys and blend order (:55-66); positions
+//                            wrap back into [0, W) exactly;
+//   pfs_ngrid_build          cell keys; a stable LSD radix sort of
+//                            (key, particle) pairs, 8-bit digits: per-block
+//                            LDS histograms, a DPP prefix scan, and a scatter
+//                            whose in-block ranks come from wavefront ballots;
+//                            cell start/end tables; positions in sorted order;
+//   pfs_ngrid_neighbors      neighbour counts, their exclusive scan, and the
+//                            lists (CSR) of every particle within radius r,
+//                            visiting the 3x3 cells in a fixed order;
+//   pfs_ngrid_density        SPH density from the lists' traversal with the
+//                            poly6 kernel, Kahan-compensated FP32 sums.
+//
+// Every integer output (keys, sort order, tables, lists) is deterministic
+// and equals the CPU restatement in oracle/pfs_particles.c bit for bit.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <string>
+
+#include "pfs.h"
+#include "pfs_device.h"
+#include "pfs_internal.h"
+
+namespace pfs {
+namespace {
+
+constexpr int kThreads = 1024;              // sort, scan and grid kernels
+constexpr int kItems = 4;                   // keys per thread
+constexpr int kTile = kThreads * kItems;    // keys per block
+constexpr int kWaves = kThreads / 64;
+constexpr int kRadix = 256;                 // 8-bit digits
+
+// ---- wavefront and block scans ------------------------------------------
+// Inclusive scan across the 64 lanes: four row shifts (1, 2, 4, 8) give
+// every 16-lane row its prefix, then the row totals are carried by the
+// row_bcast:15 (into rows 1, 3) and row_bcast:31 (into rows 2, 3) DPP forms.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return x;
+}
+
+// Exclusive scan over the block's kThreads threads; *total gets the sum.
+// Uses `sh` (kWaves + 1 words of LDS).
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t *sh, uint32_t *total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t inc = wave_incl_scan(x);
+    if (lane == 63) sh[wave] = inc;
+    __syncthreads();
+    if (wave == 0) {
+        const uint32_t w = lane < kWaves ? sh[lane] : 0u;
+        const uint32_t wi = wave_incl_scan(w);
+        if (lane < kWaves) sh[lane] = wi - w;  // exclusive over waves
+        if (lane == kWaves - 1) sh[kWaves] = wi;
+    }
+    __syncthreads();
+    const uint32_t out = sh[wave] + inc - x;
+    *total = sh[kWaves];
+    __syncthreads();  // sh may be reused by the caller
+    return out;
+}
+
+// ---- generic exclusive scan (three kernels) --------------------------------
+__global__ __launch_bounds__(kThreads) void k_scan_reduce(const uint32_t *__restrict__ in, int n,
+                                                          uint32_t *__restrict__ sums) {
+    __shared__ uint32_t sh[kWaves + 1];
+    const size_t base = (size_t)blockIdx.x * kTile + (size_t)threadIdx.x * kItems;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k)
+        if (base + k < (size_t)n) s += in[base + k];
+    uint32_t total;
+    (void)block_excl_scan(s, sh, &total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// One block: exclusive scan of sums[0, nb) in place, in chunks of kThreads.
+__global__ __launch_bounds__(kThreads) void k_scan_top(uint32_t *__restrict__ sums, int nb,
+                                                       uint32_t *__restrict__ grand) {
+    __shared__ uint32_t sh[kWaves + 1];
+    uint32_t carry = 0;
+    for (int base = 0; base < nb; base += kThreads) {
+        const int i = base + (int)threadIdx.x;
+        const uint32_t v = i < nb ? sums[i] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_excl_scan(v, sh, &total);
+        if (i < nb) sums[i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0 && grand) *grand = carry;
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_down(const uint32_t *__restrict__ in,
+                                                        uint32_t *__restrict__ out, int n,
+                                                        const uint32_t *__restrict__ sums) {
+    __shared__ uint32_t sh[kWaves + 1];
+    const size_t base = (size_t)blockIdx.x * kTile + (size_t)threadIdx.x * kItems;
+    uint32_t v[kItems], s = 0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        v[k] = base + k < (size_t)n ? in[base + k] : 0u;
+        s += v[k];
+    }
+    uint32_t total;
+    uint32_t run = sums[blockIdx.x] + block_excl_scan(s, sh, &total);
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        if (base + k < (size_t)n) out[base + k] = run;
+        run += v[k];
+    }
+}
+
+int nblocks_for(size_t n) { return (int)((n + kTile - 1) / kTile); }
+
+// Exclusive scan of n values; grand (device, may be null) gets the total.
+// ws: nblocks_for(n) words.
+int scan_exclusive(const uint32_t *in, uint32_t *out, int n, uint32_t *grand, uint32_t *ws,
+                   hipStream_t s) {
+    const int nb = nblocks_for((size_t)n);
+    if (nb == 0) return PFS_OK;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kThreads), 0, s, in, n, ws);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kThreads), 0, s, ws, nb, grand);
+    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kThreads), 0, s, in, out, n, ws);
+    return check_launch("exclusive scan");
+}
+
+// ---- stable LSD radix sort of (key, value) pairs ---------------------------
+__global__ __launch_bounds__(kThreads) void k_radix_hist(const uint32_t *__restrict__ keys, int n,
+                                                         int shift, uint32_t *__restrict__ hist,
+                                                         int nb) {
+    __shared__ uint32_t h[kRadix];
+    if (threadIdx.x < kRadix) h[threadIdx.x] = 0;
+    __syncthreads();
+    const size_t base = (size_t)blockIdx.x * kTile;
+#pragma unroll
+    for (int r = 0; r < kItems; ++r) {
+        const size_t i = base + (size_t)r * kThreads + threadIdx.x;
+        if (i < (size_t)n) atomicAdd(&h[(keys[i] >> shift) & (kRadix - 1)], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < kRadix) hist[(size_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+// Scatter of one pass.  Items are taken in tile order (round r covers
+// [r*kThreads, (r+1)*kThreads) of the block's tile), so the rank of an item
+// among equal digits is: the digit's offset for this block (the scan of the
+// digit-major histogram), plus the equal digits of earlier rounds and earlier
+// waves (LDS counts), plus the equal-digit lanes below it in its wave
+// (eight ballots, one per digit bit, intersected).
+__global__ __launch_bounds__(kThreads) void k_radix_scatter(
+    const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin, uint32_t *__restrict__ kout,
+    uint32_t *__restrict__ vout, int n, int shift, const uint32_t *__restrict__ offsets, int nb) {
+    __shared__ uint32_t wcount[kWaves][kRadix];
+    __shared__ uint32_t run[kRadix], gofs[kRadix];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x < kRadix) {
+        gofs[threadIdx.x] = offsets[(size_t)threadIdx.x * nb + blockIdx.x];
+        run[threadIdx.x] = 0;
+    }
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const size_t base = (size_t)blockIdx.x * kTile;
+    for (int r = 0; r < kItems; ++r) {
+        for (int e = threadIdx.x; e < kWaves * kRadix; e += kThreads) (&wcount[0][0])[e] = 0;
+        const size_t i = base + (size_t)r * kThreads + threadIdx.x;
+        const bool valid = i < (size_t)n;
+        const uint32_t key = valid ? kin[i] : 0u, val = valid ? vin[i] : 0u;
+        const uint32_t d = (key >> shift) & (kRadix - 1);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t m = __ballot(valid && bit);
+            peers &= bit ? m : ~m;
+        }
+        if (!valid) peers = 0;
+        const uint32_t rank = (uint32_t)__popcll(peers & below);
+        __syncthreads();  // wcount cleared; previous round's reads done
+        if (valid && rank == 0) wcount[wave][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (threadIdx.x < kRadix) {
+            uint32_t s = run[threadIdx.x];
+            for (int w = 0; w < kWaves; ++w) {
+                const uint32_t c = wcount[w][threadIdx.x];
+                wcount[w][threadIdx.x] = s;
+                s += c;
+            }
+            run[threadIdx.x] = s;
+        }
+        __syncthreads();
+        if (valid) {
+            const uint32_t pos = gofs[d] + wcount[wave][d] + rank;
+            if (pos < (uint32_t)n) {  // always, when the scan is right: kept as a bound
+                kout[pos] = key;
+                vout[pos] = val;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---- grid ------------------------------------------------------------------
+__global__ void k_fill_u32(uint32_t *__restrict__ p, size_t n, uint32_t v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_iota(uint32_t *__restrict__ p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (uint32_t)i;
+}
+
+// Cell of a position in [0, W): (int)(x * kx) with kx = gx / W (float),
+// clamped into the grid.
+__device__ __forceinline__ int cell_of(float x, float k, int g) {
+    const int c = (int)(x * k);
+    return c < 0 ? 0 : (c >= g ? g - 1 : c);
+}
+
+__global__ void k_cell_keys(const float *__restrict__ x, const float *__restrict__ y, int n,
+                            float kx, float ky, int gx, int gy, uint32_t *__restrict__ keys) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = (uint32_t)(cell_of(y[i], ky, gy) * gx + cell_of(x[i], kx, gx));
+}
+
+__global__ void k_cell_bounds(const uint32_t *__restrict__ sk, int n, uint32_t ncells,
+                              uint32_t *__restrict__ start, uint32_t *__restrict__ end) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t c = sk[p];
+    if (c >= ncells) return;  // keys are cells by construction: kept as a bound
+    if (p == 0 || sk[p - 1] != c) start[c] = (uint32_t)p;
+    if (p == n - 1 || sk[p + 1] != c) end[c] = (uint32_t)p + 1;
+}
+
+__global__ void k_gather_xy(const uint32_t *__restrict__ order, const float *__restrict__ x,
+                            const float *__restrict__ y, float *__restrict__ xs,
+                            float *__restrict__ ys, int n) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t i = order[p];
+    if (i >= (uint32_t)n) return;  // a permutation by construction: kept as a bound
+    xs[p] = x[i];
+    ys[p] = y[i];
+}
+
+// Periodic displacement on an axis of length L (half = L/2).
+__device__ __forceinline__ float periodic(float d, float L, float half) {
+    return d > half ? d - L : (d < -half ? d + L : d);
+}
+
+struct NArgs {
+    const float *xs, *ys;
+    const uint32_t *order, *sk, *start, *end;
+    int n, gx, gy;
+    float W, H, half_w, half_h, r2;
+};
+
+// Visits every particle q != p of the 3x3 cells around p's cell (rows then
+// columns, -1..1, then q ascending) within radius r: f(q, d2).
+template <typename F>
+__device__ __forceinline__ void for_neighbors(const NArgs &a, int p, F &&f) {
+    const uint32_t c = a.sk[p];
+    if (c >= (uint32_t)a.gx * (uint32_t)a.gy) return;  // a cell by construction: kept as a bound
+    const int cx = (int)(c % (uint32_t)a.gx), cy = (int)(c / (uint32_t)a.gx);
+    const float px = a.xs[p], py = a.ys[p];
+    for (int oy = -1; oy <= 1; ++oy) {
+        const int ny = cy + oy < 0 ? cy + oy + a.gy : (cy + oy >= a.gy ? cy + oy - a.gy : cy + oy);
+        for (int ox = -1; ox <= 1; ++ox) {
+            const int nx = cx + ox < 0 ? cx + ox + a.gx : (cx + ox >= a.gx ? cx + ox - a.gx : cx + ox);
+            const uint32_t nc = (uint32_t)(ny * a.gx + nx);
+            const uint32_t q1 = min(a.end[nc], (uint32_t)a.n);
+            for (uint32_t q = a.start[nc]; q < q1; ++q) {
+                if ((int)q == p) continue;
+                const float dx = periodic(a.xs[q] - px, a.W, a.half_w);
+                const float dy = periodic(a.ys[q] - py, a.H, a.half_h);
+                const float d2 = dx * dx + dy * dy;
+                if (d2 < a.r2) f(q, d2);
+            }
+        }
+    }
+}
+
+__global__ void k_nbr_count(NArgs a, uint32_t *__restrict__ counts) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n) return;
+    uint32_t c = 0;
+    for_neighbors(a, p, [&](uint32_t, float) { ++c; });
+    const uint32_t i = a.order[p];
+    if (i < (uint32_t)a.n) counts[i] = c;
+}
+
+__global__ void k_nbr_fill(NArgs a, const uint32_t *__restrict__ offsets, uint32_t *__restrict__ nbr,
+                           size_t capacity) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n) return;
+    const uint32_t i = a.order[p];
+    if (i >= (uint32_t)a.n) return;
+    size_t o = offsets[i];
+    for_neighbors(a, p, [&](uint32_t q, float) {
+        if (o < capacity) nbr[o] = a.order[q];
+        ++o;
+    });
+}
+
+// SPH density with the 2D poly6 kernel, W(d) = c (r^2 - d^2)^3, summed with
+// Kahan compensation in the traversal order, self term first.
+__global__ void k_density(NArgs a, float mass, float coef, float *__restrict__ rho) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n) return;
+    float sum = 0.0f, comp = 0.0f;
+    auto add = [&](float d2) {
+        const float t = a.r2 - d2;
+        const float w = mass * (coef * (t * t * t));
+        const float yk = w - comp;
+        const float s = sum + yk;
+        comp = (s - sum) - yk;
+        sum = s;
+    };
+    add(0.0f);
+    for_neighbors(a, p, [&](uint32_t, float d2) { add(d2); });
+    const uint32_t i = a.order[p];
+    if (i < (uint32_t)a.n) rho[i] = sum;
+}
+
+// A position wrapped into [0, n): unchanged when already inside (the
+// reference's fmod(fmod(x,n)+n,n) would re-round it), else by the exact
+// fmod; a tiny negative x whose x + n rounds to n becomes 0.
+__device__ __forceinline__ float wrap_pos(float x, float n) {
+    if (x >= 0.0f && x < n) return x;
+    x = fmodf(x, n);
+    if (x < 0.0f) x += n;
+    return x >= n ? 0.0f : x;
+}
+
+// Tracers: one forward step through the velocity field, sampled bilinearly
+// at the position: corners i0 = (int)x, i1 = (i0 + 1) mod W, weights and
+// blend as src/fluid.cpp:55-66.
+__global__ void k_tracer_advect(float *__restrict__ x, float *__restrict__ y, int n,
+                                const float *__restrict__ vp, int w, int h, int vc, float dt,
+                                float fw, float fh, float inv_w, float inv_h, int pow2_w,
+                                int pow2_h) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float px = x[i], py = y[i];
+    const int i0 = min(max((int)px, 0), w - 1), j0 = min(max((int)py, 0), h - 1);
+    const int i1 = i0 + 1 == w ? 0 : i0 + 1, j1 = j0 + 1 == h ? 0 : j0 + 1;
+    const float sx = px - (float)i0, sy = py - (float)j0;
+    const float *f00 = vp + ((size_t)j0 * w + i0) * vc, *f10 = vp + ((size_t)j0 * w + i1) * vc;
+    const float *f01 = vp + ((size_t)j1 * w + i0) * vc, *f11 = vp + ((size_t)j1 * w + i1) * vc;
+    const float u = bilerp(sx, sy, f00[0], f10[0], f01[0], f11[0]);
+    const float v = bilerp(sx, sy, f00[1], f10[1], f01[1], f11[1]);
+    x[i] = wrap_pos(px + div_n(dt * u, fw, inv_w, pow2_w), fw);
+    y[i] = wrap_pos(py + div_n(dt * v, fh, inv_h, pow2_h), fh);
+}
+
+int grid_for(int n) { return (n + 255) / 256; }
+
+int passes_for(uint32_t ncells) {
+    int bits = 0;
+    while (bits < 32 && (ncells - 1) >> bits) ++bits;
+    const int p = (bits + 7) / 8;
+    return p < 1 ? 1 : p;
+}
+
+// Workspace layout (words): sorted keys (2 buffers), values spare buffer,
+// digit histogram and its scan (kRadix * nblocks each), scan block sums.
+struct SortWs {
+    uint32_t *k0, *k1, *v1, *hist, *hscan, *sums;
+};
+size_t sort_ws_words(int n) {
+    const size_t nb = (size_t)nblocks_for((size_t)n);
+    const size_t hist = (size_t)kRadix * (nb ? nb : 1);
+    return 3 * (size_t)n + 2 * hist + (size_t)nblocks_for(hist) + (size_t)nblocks_for((size_t)n) + 64;
+}
+SortWs sort_ws(void *ws, int n) {
+    uint32_t *p = static_cast<uint32_t *>(ws);
+    const size_t nb = (size_t)nblocks_for((size_t)n);
+    const size_t hist = (size_t)kRadix * (nb ? nb : 1);
+    SortWs s;
+    s.k0 = p;
+    s.k1 = p + n;
+    s.v1 = p + 2 * (size_t)n;
+    s.hist = p + 3 * (size_t)n;
+    s.hscan = s.hist + hist;
+    s.sums = s.hscan + hist;
+    return s;
+}
+
+bool valid_grid(const pfs_ngrid *g) {
+    return g && g->n >= 1 && g->gx >= 3 && g->gy >= 3 && g->domain_w > 0 && g->domain_h > 0 &&
+           g->radius > 0 && (int64_t)g->gx * g->gy <= ((int64_t)1 << 30) &&
+           g->radius <= g->domain_w / g->gx && g->radius <= g->domain_h / g->gy;
+}
+
+NArgs nargs(const pfs_ngrid *g, const float *xs, const float *ys, const uint32_t *order,
+            const uint32_t *sk, const uint32_t *start, const uint32_t *end) {
+    NArgs a{};
+    a.xs = xs;
+    a.ys = ys;
+    a.order = order;
+    a.sk = sk;
+    a.start = start;
+    a.end = end;
+    a.n = g->n;
+    a.gx = g->gx;
+    a.gy = g->gy;
+    a.W = g->domain_w;
+    a.H = g->domain_h;
+    a.half_w = g->domain_w * 0.5f;
+    a.half_h = g->domain_h * 0.5f;
+    a.r2 = g->radius * g->radius;
+    return a;
+}
+
+}  // namespace
+}  // namespace pfs
+
+using namespace pfs;
+
+extern "C" {
+
+int pfs_ngrid_init(pfs_ngrid *g, int n, float domain_w, float domain_h, float radius) {
+    if (!g || n < 1 || !(domain_w > 0) || !(domain_h > 0) || !(radius > 0))
+        return set_error(PFS_ERR_INVALID, "ngrid: need n >= 1 and positive sizes");
+    g->n = n;
+    g->domain_w = domain_w;
+    g->domain_h = domain_h;
+    g->radius = radius;
+    const double gx = std::floor((double)domain_w / radius), gy = std::floor((double)domain_h / radius);
+    if (gx < 3 || gy < 3 || gx * gy > (double)(1 << 30))
+        return set_error(PFS_ERR_INVALID, "ngrid: the domain must span 3 to 2^30 cells of the radius");
+    g->gx = (int)gx;
+    g->gy = (int)gy;
+    // Cells at least `radius` wide, so neighbours within it are in the 3x3 block.
+    while (g->radius > g->domain_w / g->gx) --g->gx;
+    while (g->radius > g->domain_h / g->gy) --g->gy;
+    if (g->gx < 3 || g->gy < 3) return set_error(PFS_ERR_INVALID, "ngrid: fewer than 3 cells");
+    g->kx = (float)g->gx / domain_w;
+    g->ky = (float)g->gy / domain_h;
+    return PFS_OK;
+}
+
+size_t pfs_ngrid_workspace_bytes(const pfs_ngrid *g) {
+    if (!g || g->n < 1) return 0;
+    return sort_ws_words(g->n) * sizeof(uint32_t);
+}
+
+int pfs_scan_exclusive_u32(const uint32_t *in, uint32_t *out, int n, uint32_t *total, void *ws,
+                           size_t ws_bytes, void *stream) {
+    if (n < 0 || (n > 0 && (!in || !out))) return set_error(PFS_ERR_INVALID, "scan: bad arguments");
+    if (!ws || ws_bytes < (size_t)(nblocks_for((size_t)n) + 1) * sizeof(uint32_t))
+        return set_error(PFS_ERR_INVALID, "scan: workspace too small");
+    return scan_exclusive(in, out, n, total, static_cast<uint32_t *>(ws), (hipStream_t)stream);
+}
+
+int pfs_particles_advect(float *x, float *y, int n, const pfs_field *vp, float dt, void *stream) {
+    PFS_TRY(check_field(vp, "vp", 2));
+    if (n < 0 || (n > 0 && (!x || !y))) return set_error(PFS_ERR_INVALID, "advect: bad particles");
+    if (n == 0) return PFS_OK;
+    hipLaunchKernelGGL(k_tracer_advect, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, x, y,
+                       n, vp->data, vp->x, vp->y, vp->z, dt, (float)vp->x, (float)vp->y,
+                       1.0f / (float)vp->x, 1.0f / (float)vp->y, (vp->x & (vp->x - 1)) == 0,
+                       (vp->y & (vp->y - 1)) == 0);
+    return check_launch("particle advect");
+}
+
+int pfs_ngrid_build(const pfs_ngrid *g, const float *x, const float *y, uint32_t *keys,
+                    uint32_t *order, uint32_t *cell_start, uint32_t *cell_end, float *xs, float *ys,
+                    void *ws, size_t ws_bytes, void *stream) {
+    if (!valid_grid(g)) return set_error(PFS_ERR_INVALID, "ngrid: not initialised by pfs_ngrid_init");
+    if (!x || !y || !keys || !order || !cell_start || !cell_end || !xs || !ys)
+        return set_error(PFS_ERR_INVALID, "ngrid: NULL buffer");
+    if (!ws || ws_bytes < pfs_ngrid_workspace_bytes(g))
+        return set_error(PFS_ERR_INVALID, "ngrid: workspace smaller than pfs_ngrid_workspace_bytes()");
+    hipStream_t s = (hipStream_t)stream;
+    const int n = g->n, nb = nblocks_for((size_t)n);
+    const uint32_t ncells = (uint32_t)g->gx * (uint32_t)g->gy;
+    SortWs w = sort_ws(ws, n);
+    hipLaunchKernelGGL(k_cell_keys, dim3(grid_for(n)), dim3(256), 0, s, x, y, n, g->kx, g->ky,
+                       g->gx, g->gy, keys);
+    // Values start as particle ids in the buffer that makes the last pass
+    // land in `order`.
+    // Buffers alternate so that the last pass writes the keys to k0 (kept for
+    // the neighbour passes) and the particle ids to `order`.
+    const int passes = passes_for(ncells);
+    uint32_t *kb[2] = {passes % 2 == 0 ? w.k0 : w.k1, passes % 2 == 0 ? w.k1 : w.k0};
+    uint32_t *vb[2] = {passes % 2 == 0 ? order : w.v1, passes % 2 == 0 ? w.v1 : order};
+    hipLaunchKernelGGL(k_iota, dim3(grid_for(n)), dim3(256), 0, s, vb[0], n);
+    const uint32_t *kin = keys;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = 8 * p;
+        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kThreads), 0, s, kin, n, shift, w.hist, nb);
+        PFS_TRY(scan_exclusive(w.hist, w.hscan, kRadix * nb, nullptr, w.sums, s));
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kThreads), 0, s, kin, vb[p & 1],
+                           kb[(p + 1) & 1], vb[(p + 1) & 1], n, shift, w.hscan, nb);
+        kin = kb[(p + 1) & 1];
+    }
+    const uint32_t *sk = kin;
+    const int fill_blocks = (int)std::min<size_t>(((size_t)ncells + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill_u32, dim3(fill_blocks), dim3(256), 0, s, cell_start, (size_t)ncells, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(fill_blocks), dim3(256), 0, s, cell_end, (size_t)ncells, 0u);
+    hipLaunchKernelGGL(k_cell_bounds, dim3(grid_for(n)), dim3(256), 0, s, sk, n, ncells, cell_start,
+                       cell_end);
+    hipLaunchKernelGGL(k_gather_xy, dim3(grid_for(n)), dim3(256), 0, s, order, x, y, xs, ys, n);
+    return check_launch("neighbour grid");
+}
+
+int pfs_ngrid_neighbors(const pfs_ngrid *g, const float *xs, const float *ys, const uint32_t *order,
+                        const uint32_t *cell_start, const uint32_t *cell_end, uint32_t *counts,
+                        uint32_t *offsets, uint32_t *nbr, size_t nbr_capacity, uint32_t *total,
+                        void *ws, size_t ws_bytes, void *stream) {
+    if (!valid_grid(g)) return set_error(PFS_ERR_INVALID, "ngrid: not initialised by pfs_ngrid_init");
+    if (!xs || !ys || !order || !cell_start || !cell_end || !counts || !offsets || !total)
+        return set_error(PFS_ERR_INVALID, "ngrid: NULL buffer");
+    if (!ws || ws_bytes < pfs_ngrid_workspace_bytes(g))
+        return set_error(PFS_ERR_INVALID, "ngrid: workspace smaller than pfs_ngrid_workspace_bytes()");
+    hipStream_t s = (hipStream_t)stream;
+    SortWs w = sort_ws(ws, g->n);
+    const NArgs a = nargs(g, xs, ys, order, w.k0, cell_start, cell_end);
+    hipLaunchKernelGGL(k_nbr_count, dim3(grid_for(g->n)), dim3(256), 0, s, a, counts);
+    PFS_TRY(scan_exclusive(counts, offsets, g->n, total, w.sums, s));
+    if (!nbr) return PFS_OK;  // counts and offsets only
+    uint32_t h_total = 0;
+    PFS_HIP_TRY(hipMemcpyAsync(&h_total, total, sizeof h_total, hipMemcpyDeviceToHost, s));
+    PFS_HIP_TRY(hipStreamSynchronize(s));
+    if (h_total > nbr_capacity)
+        return set_error(PFS_ERR_INVALID, "ngrid: neighbour list capacity " +
+                                              std::to_string(nbr_capacity) + " < " +
+                                              std::to_string(h_total));
+    hipLaunchKernelGGL(k_nbr_fill, dim3(grid_for(g->n)), dim3(256), 0, s, a, offsets, nbr,
+                       nbr_capacity);
+    return check_launch("neighbour lists");
+}
+
+int pfs_ngrid_density(const pfs_ngrid *g, const float *xs, const float *ys, const uint32_t *order,
+                      const uint32_t *cell_start, const uint32_t *cell_end, float mass, float *rho,
+                      void *ws, size_t ws_bytes, void *stream) {
+    if (!valid_grid(g)) return set_error(PFS_ERR_INVALID, "ngrid: not initialised by pfs_ngrid_init");
+    if (!xs || !ys || !order || !cell_start || !cell_end || !rho)
+        return set_error(PFS_ERR_INVALID, "ngrid: NULL buffer");
+    if (!ws || ws_bytes < pfs_ngrid_workspace_bytes(g))
+        return set_error(PFS_ERR_INVALID, "ngrid: workspace smaller than pfs_ngrid_workspace_bytes()");
+    SortWs w = sort_ws(ws, g->n);
+    const NArgs a = nargs(g, xs, ys, order, w.k0, cell_start, cell_end);
+    // 2D poly6: 4 / (pi r^8), in float from the double constant.
+    const double r = g->radius;
+    const float coef = (float)(4.0 / (3.14159265358979323846 * r * r * r * r * r * r * r * r));
+    hipLaunchKernelGGL(k_density, dim3(grid_for(g->n)), dim3(256), 0, (hipStream_t)stream, a, mass,
+                       coef, rho);
+    return check_launch("density");
+}
+
+}  // extern "C"

@@ -1,0 +1,118 @@
+"""Particle layer over the grid fluid: tracers and their neighbour grid.
+
+No counterpart in the reference, which has no particles.  Python face of
+pfs_particles_advect and pfs_ngrid_* (include/pfs.h): positions are float32
+device tensors in field-cell units on the periodic domain [0, W) x [0, H);
+integer outputs are int32 tensors holding the C ABI's uint32 values.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Tuple
+
+import torch
+
+from . import _lib
+from .fluid import VPField, _Bound, _stream
+
+
+PfsNGrid = _lib.PfsNGrid
+
+
+def _ptr(t: torch.Tensor) -> ctypes.c_void_p:
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _check_xy(x: torch.Tensor, y: torch.Tensor) -> None:
+    if x.dtype != torch.float32 or y.dtype != torch.float32 or x.shape != y.shape or x.dim() != 1:
+        raise ValueError("x, y must be 1-D float32 tensors of one length")
+    if not (x.is_cuda and y.is_cuda and x.is_contiguous() and y.is_contiguous()):
+        raise ValueError("x, y must be contiguous device tensors")
+
+
+def advect(x: torch.Tensor, y: torch.Tensor, vp: VPField, dt: float, stream=None) -> None:
+    """One forward step of the tracers through vp's velocity (in place)."""
+    _check_xy(x, y)
+    b = _Bound(vp)
+    _lib.call("pfs_particles_advect", _ptr(x), _ptr(y), x.numel(), b.ptr(0), dt, _stream(stream))
+
+
+def scan_exclusive(t: torch.Tensor, stream=None) -> Tuple[torch.Tensor, int]:
+    """Exclusive prefix sum of an int32 tensor read as uint32; (out, total)."""
+    if t.dtype != torch.int32 or not t.is_contiguous():
+        raise ValueError("need a contiguous int32 tensor")
+    n = t.numel()
+    out = torch.empty_like(t)
+    total = torch.zeros(1, dtype=torch.int32, device=t.device)
+    ws = torch.empty(((n + 4095) // 4096 + 1,), dtype=torch.int32, device=t.device)
+    _lib.call("pfs_scan_exclusive_u32", _ptr(t), _ptr(out), n, _ptr(total), _ptr(ws),
+              ws.numel() * 4, _stream(stream))
+    return out, int(total.item()) & 0xFFFFFFFF
+
+
+class NeighborGrid:
+    """Uniform grid of cells at least `radius` wide over n particles."""
+
+    def __init__(self, n: int, domain_w: float, domain_h: float, radius: float,
+                 device: str = "cuda"):
+        self.g = PfsNGrid()
+        _lib.call("pfs_ngrid_init", ctypes.byref(self.g), n, domain_w, domain_h, radius)
+        i32 = dict(dtype=torch.int32, device=device)
+        nc = self.g.gx * self.g.gy
+        self.keys = torch.empty(n, **i32)
+        self.order = torch.empty(n, **i32)
+        self.cell_start = torch.empty(nc, **i32)
+        self.cell_end = torch.empty(nc, **i32)
+        self.xs = torch.empty(n, dtype=torch.float32, device=device)
+        self.ys = torch.empty(n, dtype=torch.float32, device=device)
+        nbytes = _lib.load().pfs_ngrid_workspace_bytes(ctypes.byref(self.g))
+        self.ws = torch.empty((nbytes + 3) // 4, **i32)
+        self.counts = torch.empty(n, **i32)
+        self.offsets = torch.empty(n, **i32)
+        self.total = torch.zeros(1, **i32)
+
+    @property
+    def n(self) -> int:
+        return self.g.n
+
+    @property
+    def cells(self) -> Tuple[int, int]:
+        return self.g.gx, self.g.gy
+
+    def build(self, x: torch.Tensor, y: torch.Tensor, stream=None) -> None:
+        """Keys, sort order, cell tables and sorted positions for (x, y)."""
+        _check_xy(x, y)
+        if x.numel() != self.n:
+            raise ValueError("particle count differs from the grid's")
+        _lib.call("pfs_ngrid_build", ctypes.byref(self.g), _ptr(x), _ptr(y), _ptr(self.keys),
+                  _ptr(self.order), _ptr(self.cell_start), _ptr(self.cell_end), _ptr(self.xs),
+                  _ptr(self.ys), _ptr(self.ws), self.ws.numel() * 4, _stream(stream))
+
+    def _common(self):
+        return (ctypes.byref(self.g), _ptr(self.xs), _ptr(self.ys), _ptr(self.order),
+                _ptr(self.cell_start), _ptr(self.cell_end))
+
+    def neighbor_counts(self, stream=None) -> Tuple[torch.Tensor, torch.Tensor]:
+        """(counts, exclusive offsets) by particle id, without the lists."""
+        _lib.call("pfs_ngrid_neighbors", *self._common(), _ptr(self.counts), _ptr(self.offsets),
+                  None, 0, _ptr(self.total), _ptr(self.ws), self.ws.numel() * 4, _stream(stream))
+        return self.counts, self.offsets
+
+    def neighbors(self, stream=None) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+        """(counts, offsets, lists): particle i's neighbours are
+        lists[offsets[i] : offsets[i] + counts[i]].  Synchronises the stream
+        to size the lists."""
+        self.neighbor_counts(stream)
+        total = int(self.total.item()) & 0xFFFFFFFF
+        nbr = torch.empty(max(total, 1), dtype=torch.int32, device=self.keys.device)
+        _lib.call("pfs_ngrid_neighbors", *self._common(), _ptr(self.counts), _ptr(self.offsets),
+                  _ptr(nbr), nbr.numel(), _ptr(self.total), _ptr(self.ws), self.ws.numel() * 4,
+                  _stream(stream))
+        return self.counts, self.offsets, nbr[:total]
+
+    def density(self, mass: float, stream=None) -> torch.Tensor:
+        """SPH (2D poly6) density per particle id."""
+        rho = torch.empty(self.n, dtype=torch.float32, device=self.keys.device)
+        _lib.call("pfs_ngrid_density", *self._common(), mass, _ptr(rho), _ptr(self.ws),
+                  self.ws.numel() * 4, _stream(stream))
+        return rho

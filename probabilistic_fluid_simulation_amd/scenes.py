@@ -276,3 +276,48 @@ def served_specs(n: int, rate: float, seed: int = 7) -> List[dict]:
         out.append(dict(job=k, arrival=t, fw=side, fh=side, iw=side, ih=side, steps=steps,
                         seed=k))
     return out
+
+
+# ---- particle sets (the particle layer; no reference counterpart) -----------
+def particles_disk(n: int, w: int, h: int, seed: int = 0, radius: float = 0.3):
+    """n positions uniform in the disk of radius `radius` (domain units)
+    centred in the domain, in field-cell units: (x, y) float32 arrays."""
+    r = uniform(seed, 2 * n, stream=6).reshape(n, 2)
+    rad = radius * np.sqrt(r[:, 0])
+    ang = 2.0 * np.pi * r[:, 1]
+    x = (0.5 + rad * np.cos(ang)) * w
+    y = (0.5 + rad * np.sin(ang)) * h
+    return x.astype(np.float32), y.astype(np.float32)
+
+
+def particles_in_mask(mask: np.ndarray, n: int, seed: int = 0):
+    """n positions uniform over the set pixels of an (H, W) mask: a set pixel
+    drawn uniformly, then a uniform point inside it (field-cell units)."""
+    idx = np.flatnonzero(mask.reshape(-1) > 0)
+    if idx.size == 0:
+        raise ValueError("empty mask")
+    h, w = mask.shape
+    r = uniform(seed, 3 * n, stream=7).reshape(n, 3)
+    pick = idx[np.minimum((r[:, 0] * idx.size).astype(np.int64), idx.size - 1)]
+    x = ((pick % w) + r[:, 1]).astype(np.float32)
+    y = ((pick // w) + r[:, 2]).astype(np.float32)
+    # float32 rounding may reach the far edge: keep positions inside [0, W)
+    x = np.minimum(x, np.nextafter(np.float32(w), np.float32(0)))
+    y = np.minimum(y, np.nextafter(np.float32(h), np.float32(0)))
+    return x, y
+
+
+def particles_16k():
+    """16,384 particles in the disk of radius 0.3 on the 128x128 Taylor-Green
+    field of the 16K workload; radius 0.02 of the domain (2.56 cells)."""
+    s = plain_16k()
+    x, y = particles_disk(16384, 128, 128, seed=0)
+    return s, x, y, 0.02 * 128
+
+
+def particles_1m():
+    """1,048,576 particles over the set pixels of the 1M workload's dye blobs,
+    on its 1024x1024 vortex field; radius 1.5 cells."""
+    s = plain_1m()
+    x, y = particles_in_mask(s.image[..., 0], 1 << 20, seed=0)
+    return s, x, y, 1.5
