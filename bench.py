@@ -216,6 +216,100 @@ def precision_leg(steps: int, sample_every: int, side: int = 2048) -> dict:
             "error_samples": len(err_c)}
 
 
+def particles_leg(steps: int, warmup: int = 5) -> dict:
+    """The particle layer (no reference counterpart) at 1,048,576 tracers
+    seeded over the 1M workload's dye blobs, radius 1.5 cells on its
+    1024x1024 vortex field: each step advects the tracers through the field,
+    rebuilds the neighbour grid (keys, radix sort, cell tables, sorted
+    positions), counts neighbours and sums densities.  Timed with HIP events
+    per stage; afterwards the final grid, lists and densities are checked
+    bit-exact against the CPU oracle on the same positions."""
+    import numpy as np
+    import torch
+
+    from oracle import particles as op
+    from probabilistic_fluid_simulation_amd import fluid, particles, scenes
+
+    s, x, y, r = scenes.particles_1m()
+    n = len(x)
+    vp, tmp = fluid.VPField.from_numpy(s.vp), fluid.VPField.from_numpy(s.tmp)
+    im, itmp = fluid.VPField.from_numpy(s.image), fluid.VPField.from_numpy(s.itmp)
+    xg, yg = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()
+    ng = particles.NeighborGrid(n, float(s.vp.shape[1]), float(s.vp.shape[0]), r)
+    mass = 1.0 / n
+    stages = ("advect", "grid", "counts", "density")
+
+    def pstep(ev=None):
+        if ev:
+            ev[0].record()
+        particles.advect(xg, yg, vp, s.dt)
+        if ev:
+            ev[1].record()
+        ng.build(xg, yg)
+        if ev:
+            ev[2].record()
+        ng.neighbor_counts()
+        if ev:
+            ev[3].record()
+        rho = ng.density(mass)
+        if ev:
+            ev[4].record()
+        return rho
+
+    for _ in range(warmup):
+        pstep()
+    torch.cuda.synchronize()
+    per = {k: 0.0 for k in stages}
+    for _ in range(steps):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+        pstep(ev)
+        torch.cuda.synchronize()
+        for k, name in enumerate(stages):
+            per[name] += ev[k].elapsed_time(ev[k + 1])
+    per = {k: v / steps for k, v in per.items()}
+    # The whole layer back to back, and with the grid fluid step beside it.
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(steps):
+        pstep()
+    e1.record()
+    torch.cuda.synchronize()
+    layer_ms = e0.elapsed_time(e1) / steps
+    e0.record()
+    for _ in range(steps):
+        fluid.fluid_timestep(vp, tmp, im, itmp, s.dt, s.visc, s.iters)
+        pstep()
+    e1.record()
+    torch.cuda.synchronize()
+    both_ms = e0.elapsed_time(e1) / steps
+    # bit-exact check on the final positions
+    rho = pstep()
+    counts, offsets, nbr = ng.neighbors()
+    torch.cuda.synchronize()
+    u32 = lambda t: t.cpu().numpy().view(np.uint32)  # noqa: E731
+    xo, yo = xg.cpu().numpy(), yg.cpu().numpy()
+    og = op.Grid(xo, yo, ng.g.domain_w, ng.g.domain_h, r, ng.g.gx, ng.g.gy, ng.g.kx, ng.g.ky)
+    oc, oo, on = og.neighbors()
+    exact = (np.array_equal(u32(ng.keys), og.keys) and np.array_equal(u32(ng.order), og.order)
+             and np.array_equal(u32(ng.cell_start), og.start)
+             and np.array_equal(u32(ng.cell_end), og.end)
+             and np.array_equal(u32(counts), oc) and np.array_equal(u32(offsets), oo)
+             and np.array_equal(u32(nbr), on)
+             and rho.cpu().numpy().tobytes() == og.density(mass).tobytes())
+    total = int(oc.sum())
+    return {"workload": "particles-1M (1,048,576 tracers over the 1024x1024 blob image, "
+                        "radius 1.5 cells)",
+            "particles": n, "cells": ng.g.gx * ng.g.gy, "steps": steps,
+            "neighbors_per_particle": total / n,
+            "stage_ms": per, "layer_ms_per_step": layer_ms,
+            "particle_steps_per_s": n / (layer_ms * 1e-3),
+            "with_fluid_step_ms": both_ms,
+            "bit_exact_vs_oracle": bool(exact),
+            "checked": "keys, sort order, cell start/end, neighbour counts, offsets, lists and "
+                       "densities of the final step"}
+
+
 def served_leg(rate: float, duration: float, seed: int = 7) -> dict:
     """The served workload: Poisson arrivals at `rate` jobs/s for `duration`
     seconds (seed 7), dye pixels log-uniform in [8192, 2^20], field side =
@@ -245,6 +339,8 @@ def main() -> int:
     ap.add_argument("--batched-steps", type=int, default=300)
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--no-particles", action="store_true")
+    ap.add_argument("--particle-steps", type=int, default=50)
     ap.add_argument("--precision-steps", type=int, default=20)
     ap.add_argument("--served-rates", default="64",
                     help="comma-separated arrival rates (jobs/s) of the served leg")
@@ -356,6 +452,8 @@ def main() -> int:
     if not args.no_served:
         out["served"] = [served_leg(float(r), args.served_seconds)
                          for r in args.served_rates.split(",")]
+    if not args.no_particles:
+        out["particles"] = particles_leg(args.particle_steps)
     if not args.no_precision:
         out["precision"] = precision_leg(args.precision_steps, 5)
     print(json.dumps(out), flush=True)
