@@ -219,11 +219,13 @@ def precision_leg(steps: int, sample_every: int, side: int = 2048) -> dict:
 def particles_leg(steps: int, warmup: int = 5) -> dict:
     """The particle layer (no reference counterpart) at 1,048,576 tracers
     seeded over the 1M workload's dye blobs, radius 1.5 cells on its
-    1024x1024 vortex field: each step advects the tracers through the field,
-    rebuilds the neighbour grid (keys, radix sort, cell tables, sorted
-    positions), counts neighbours and sums densities.  Timed with HIP events
-    per stage; afterwards the final grid, lists and densities are checked
-    bit-exact against the CPU oracle on the same positions."""
+    1024x1024 vortex field.  The tracers are kept in cell order
+    (particles.Tracers): each step advects them through the field, rebuilds
+    the neighbour grid (keys, radix sort, cell tables) and adopts its order,
+    counts neighbours and sums densities.  Timed with HIP events per stage;
+    then one more step is replayed by the CPU oracle from the same state and
+    positions, ids, order, cell tables, neighbour lists and densities are
+    compared bit for bit."""
     import numpy as np
     import torch
 
@@ -232,36 +234,34 @@ def particles_leg(steps: int, warmup: int = 5) -> dict:
 
     s, x, y, r = scenes.particles_1m()
     n = len(x)
+    H, W = s.vp.shape[:2]
     vp, tmp = fluid.VPField.from_numpy(s.vp), fluid.VPField.from_numpy(s.tmp)
     im, itmp = fluid.VPField.from_numpy(s.image), fluid.VPField.from_numpy(s.itmp)
-    xg, yg = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()
-    ng = particles.NeighborGrid(n, float(s.vp.shape[1]), float(s.vp.shape[0]), r)
+    tr = particles.Tracers(torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda(), float(W),
+                           float(H), r)
     mass = 1.0 / n
-    stages = ("advect", "grid", "counts", "density")
+    rho = torch.empty(n, dtype=torch.float32, device="cuda")
+    stages = ("advect", "grid", "density_and_counts")
 
     def pstep(ev=None):
         if ev:
             ev[0].record()
-        particles.advect(xg, yg, vp, s.dt)
+        tr.advect(vp, s.dt)
         if ev:
             ev[1].record()
-        ng.build(xg, yg)
+        tr.sort()
         if ev:
             ev[2].record()
-        ng.neighbor_counts()
+        tr.density(mass, out=rho, with_counts=True)
         if ev:
             ev[3].record()
-        rho = ng.density(mass)
-        if ev:
-            ev[4].record()
-        return rho
 
     for _ in range(warmup):
         pstep()
     torch.cuda.synchronize()
     per = {k: 0.0 for k in stages}
     for _ in range(steps):
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(stages) + 1)]
         pstep(ev)
         torch.cuda.synchronize()
         for k, name in enumerate(stages):
@@ -283,31 +283,38 @@ def particles_leg(steps: int, warmup: int = 5) -> dict:
     e1.record()
     torch.cuda.synchronize()
     both_ms = e0.elapsed_time(e1) / steps
-    # bit-exact check on the final positions
-    rho = pstep()
-    counts, offsets, nbr = ng.neighbors()
+    # One more step, replayed by the oracle from the same state.
+    xo, yo = tr.x.cpu().numpy().copy(), tr.y.cpu().numpy().copy()
+    ids = tr.ids.cpu().numpy().view(np.uint32).copy()
+    field = vp.numpy()
+    pstep()
+    fused_counts = tr.grid.counts.clone()
+    counts, offsets, nbr = tr.neighbors()
     torch.cuda.synchronize()
     u32 = lambda t: t.cpu().numpy().view(np.uint32)  # noqa: E731
-    xo, yo = xg.cpu().numpy(), yg.cpu().numpy()
-    og = op.Grid(xo, yo, ng.g.domain_w, ng.g.domain_h, r, ng.g.gx, ng.g.gy, ng.g.kx, ng.g.ky)
-    oc, oo, on = og.neighbors()
-    exact = (np.array_equal(u32(ng.keys), og.keys) and np.array_equal(u32(ng.order), og.order)
-             and np.array_equal(u32(ng.cell_start), og.start)
-             and np.array_equal(u32(ng.cell_end), og.end)
+    op.advect(xo, yo, field, s.dt)
+    g = tr.grid.g
+    og = op.Grid(xo, yo, g.domain_w, g.domain_h, r, g.gx, g.gy, g.kx, g.ky)
+    oc, oo, on = og.neighbors(by_position=True)
+    exact = (tr.x.cpu().numpy().tobytes() == og.xs.tobytes()
+             and tr.y.cpu().numpy().tobytes() == og.ys.tobytes()
+             and np.array_equal(u32(tr.ids), ids[og.order])
+             and np.array_equal(u32(tr.grid.order), og.order)
+             and np.array_equal(u32(tr.grid.cell_start), og.start)
+             and np.array_equal(u32(tr.grid.cell_end), og.end)
              and np.array_equal(u32(counts), oc) and np.array_equal(u32(offsets), oo)
-             and np.array_equal(u32(nbr), on)
-             and rho.cpu().numpy().tobytes() == og.density(mass).tobytes())
-    total = int(oc.sum())
+             and np.array_equal(u32(fused_counts), oc) and np.array_equal(u32(nbr), on)
+             and rho.cpu().numpy().tobytes() == og.density(mass, by_position=True).tobytes())
     return {"workload": "particles-1M (1,048,576 tracers over the 1024x1024 blob image, "
                         "radius 1.5 cells)",
-            "particles": n, "cells": ng.g.gx * ng.g.gy, "steps": steps,
-            "neighbors_per_particle": total / n,
+            "particles": n, "cells": g.gx * g.gy, "steps": steps,
+            "neighbors_per_particle": int(oc.sum()) / n,
             "stage_ms": per, "layer_ms_per_step": layer_ms,
             "particle_steps_per_s": n / (layer_ms * 1e-3),
             "with_fluid_step_ms": both_ms,
             "bit_exact_vs_oracle": bool(exact),
-            "checked": "keys, sort order, cell start/end, neighbour counts, offsets, lists and "
-                       "densities of the final step"}
+            "checked": "one step replayed by the oracle: positions, ids, sort order, cell "
+                       "tables, neighbour counts, offsets, lists and densities"}
 
 
 def served_leg(rate: float, duration: float, seed: int = 7) -> dict:

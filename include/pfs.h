@@ -237,27 +237,34 @@ typedef struct pfs_ngrid {
 int pfs_particles_advect(float *x, float *y, int n, const pfs_field *vp, float dt, void *stream);
 /* Fills *g: gx = floor(W / r) (lowered until cells are >= r wide), likewise gy. */
 int pfs_ngrid_init(pfs_ngrid *g, int n, float domain_w, float domain_h, float radius);
+/* Workspaces of pfs_ngrid_* are 8-byte aligned, of this many bytes. */
 size_t pfs_ngrid_workspace_bytes(const pfs_ngrid *g);
 /* keys[n]: cell of particle i; order[n]: particle ids sorted by (key, id);
  * cell_start/cell_end[gx*gy]: [start, end) of each cell in sorted order
- * (0, 0 when empty); xs, ys[n]: positions in sorted order.  The workspace
- * keeps the sorted keys for pfs_ngrid_neighbors / pfs_ngrid_density. */
+ * (an empty cell has start = end = its lower bound, so the cells of a row
+ * are consecutive ranges); xs, ys[n]: positions in sorted order.  The
+ * workspace keeps the sorted keys for pfs_ngrid_neighbors / _density. */
 int pfs_ngrid_build(const pfs_ngrid *g, const float *x, const float *y, uint32_t *keys,
                     uint32_t *order, uint32_t *cell_start, uint32_t *cell_end, float *xs, float *ys,
                     void *workspace, size_t workspace_bytes, void *stream);
 /* counts[i]: neighbours of particle i within r (periodic distance, d^2 < r^2,
  * itself excluded); offsets[i]: their exclusive scan; *total (device): the
  * sum.  With nbr non-NULL also the lists, nbr[offsets[i] + k] (particle
- * ids); this waits for *total to check nbr_capacity. */
+ * ids); this waits for *total to check nbr_capacity.  With order NULL,
+ * particles are identified by sorted position instead of id (for callers
+ * that keep their particles in cell order, pfs_gather_u32). */
 int pfs_ngrid_neighbors(const pfs_ngrid *g, const float *xs, const float *ys, const uint32_t *order,
                         const uint32_t *cell_start, const uint32_t *cell_end, uint32_t *counts,
                         uint32_t *offsets, uint32_t *nbr, size_t nbr_capacity, uint32_t *total,
                         void *workspace, size_t workspace_bytes, void *stream);
 /* rho[i] = sum over i and its neighbours of mass * 4/(pi r^8) * (r^2 - d^2)^3
- * (2D poly6), Kahan-compensated FP32 in the neighbour order, self first. */
+ * (2D poly6), Kahan-compensated FP32 in the neighbour order, self first;
+ * counts (may be NULL) gets the neighbour counts of the same traversal. */
 int pfs_ngrid_density(const pfs_ngrid *g, const float *xs, const float *ys, const uint32_t *order,
                       const uint32_t *cell_start, const uint32_t *cell_end, float mass, float *rho,
-                      void *workspace, size_t workspace_bytes, void *stream);
+                      uint32_t *counts, void *workspace, size_t workspace_bytes, void *stream);
+/* out[p] = in[index[p]]: permutes per-particle data into sorted order. */
+int pfs_gather_u32(const uint32_t *index, const uint32_t *in, uint32_t *out, int n, void *stream);
 /* Exclusive prefix sum of n uint32 (wavefront DPP scans); *total (device,
  * may be NULL) gets the sum.  Workspace: (ceil(n / 4096) + 1) * 4 bytes. */
 int pfs_scan_exclusive_u32(const uint32_t *in, uint32_t *out, int n, uint32_t *total,

@@ -62,11 +62,13 @@ class Grid:
         assert rc == 0
         self.keys_sorted = np.ascontiguousarray(self.keys[self.order])
 
-    def neighbors(self) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
-        """(counts by id, exclusive offsets, lists)."""
+    def neighbors(self, by_position: bool = False) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """(counts by id, exclusive offsets, lists); by sorted position with
+        by_position."""
         lib = _load()
         counts = np.empty(self.n, np.uint32)
-        args = (_p(self.xs), _p(self.ys), _p(self.order), self.n, _p(self.keys_sorted),
+        order = None if by_position else _p(self.order)
+        args = (_p(self.xs), _p(self.ys), order, self.n, _p(self.keys_sorted),
                 _p(self.start), _p(self.end), self.gx, self.gy, self.W, self.H, self.r)
         total = lib.orcp_ngrid_neighbors(*args, _p(counts), None, None)
         offsets = np.zeros(self.n, np.uint32)
@@ -75,9 +77,10 @@ class Grid:
         lib.orcp_ngrid_neighbors(*args, _p(counts), _p(offsets), _p(nbr))
         return counts, offsets, nbr[:int(total)]
 
-    def density(self, mass: float) -> np.ndarray:
+    def density(self, mass: float, by_position: bool = False) -> np.ndarray:
         rho = np.empty(self.n, np.float32)
-        _load().orcp_ngrid_density(_p(self.xs), _p(self.ys), _p(self.order), self.n,
+        order = None if by_position else _p(self.order)
+        _load().orcp_ngrid_density(_p(self.xs), _p(self.ys), order, self.n,
                                    _p(self.keys_sorted), _p(self.start), _p(self.end), self.gx,
                                    self.gy, self.W, self.H, self.r, mass, _p(rho))
         return rho

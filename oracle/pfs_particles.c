@@ -63,13 +63,10 @@ int orcp_ngrid_build(const float *x, const float *y, int n, float kx, float ky, 
         count[keys[i] + 1]++;
     }
     for (size_t c = 0; c < nc; ++c) count[c + 1] += count[c];  /* count[c] = start of c */
-    memset(start, 0, nc * sizeof(uint32_t));
-    memset(end, 0, nc * sizeof(uint32_t));
-    for (size_t c = 0; c < nc; ++c)
-        if (count[c + 1] > count[c]) {
-            start[c] = count[c];
-            end[c] = count[c + 1];
-        }
+    for (size_t c = 0; c < nc; ++c) {  /* every cell: [lower bound, upper bound) */
+        start[c] = count[c];
+        end[c] = count[c + 1];
+    }
     for (int i = 0; i < n; ++i) order[count[keys[i]]++] = (uint32_t)i;
     for (int p = 0; p < n; ++p) {
         xs[p] = x[order[p]];
@@ -84,7 +81,8 @@ static float periodic(float d, float L, float half) {
 }
 
 /* Neighbours of every particle (ids in traversal order), counts by particle
- * id; nbr may be NULL (counts only).  Returns the total. */
+ * id; nbr may be NULL (counts only); order NULL: ids are sorted positions.
+ * Returns the total. */
 long long orcp_ngrid_neighbors(const float *xs, const float *ys, const uint32_t *order, int n,
                                const uint32_t *keys_sorted, const uint32_t *start,
                                const uint32_t *end, int gx, int gy, float W, float H, float r,
@@ -95,7 +93,8 @@ long long orcp_ngrid_neighbors(const float *xs, const float *ys, const uint32_t 
         const uint32_t c = keys_sorted[p];
         const int cx = (int)(c % (uint32_t)gx), cy = (int)(c / (uint32_t)gx);
         const float px = xs[p], py = ys[p];
-        uint32_t k = 0, o = (nbr && offsets) ? offsets[order[p]] : 0;
+        const uint32_t id = order ? order[p] : (uint32_t)p;
+        uint32_t k = 0, o = (nbr && offsets) ? offsets[id] : 0;
         for (int oy = -1; oy <= 1; ++oy) {
             const int ny = (cy + oy + gy) % gy;
             for (int ox = -1; ox <= 1; ++ox) {
@@ -107,13 +106,13 @@ long long orcp_ngrid_neighbors(const float *xs, const float *ys, const uint32_t 
                     const float dy = periodic(ys[q] - py, H, half_h);
                     const float d2 = dx * dx + dy * dy;
                     if (d2 < r2) {
-                        if (nbr && offsets) nbr[o + k] = order[q];
+                        if (nbr && offsets) nbr[o + k] = order ? order[q] : q;
                         ++k;
                     }
                 }
             }
         }
-        counts[order[p]] = k;
+        counts[id] = k;
         total += k;
     }
     return total;
@@ -156,6 +155,6 @@ void orcp_ngrid_density(const float *xs, const float *ys, const uint32_t *order,
             }
         }
 #undef ADD
-        rho[order[p]] = sum;
+        rho[order ? order[p] : (uint32_t)p] = sum;
     }
 }

@@ -11,7 +11,9 @@
 //                            (key, particle) pairs, 8-bit digits: per-block
 //                            LDS histograms, a DPP prefix scan, and a scatter
 //                            whose in-block ranks come from wavefront ballots;
-//                            cell start/end tables; positions in sorted order;
+//                            cell start/end tables (the scan of the cell
+//                            counts: every cell, empty or not, gets its
+//                            lower bound); positions in sorted order;
 //   pfs_ngrid_neighbors      neighbour counts, their exclusive scan, and the
 //                            lists (CSR) of every particle within radius r,
 //                            visiting the 3x3 cells in a fixed order;
@@ -214,17 +216,6 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
 }
 
 // ---- grid ------------------------------------------------------------------
-__global__ void k_fill_u32(uint32_t *__restrict__ p, size_t n, uint32_t v) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x)
-        p[i] = v;
-}
-
-__global__ void k_iota(uint32_t *__restrict__ p, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = (uint32_t)i;
-}
-
 // Cell of a position in [0, W): (int)(x * kx) with kx = gx / W (float),
 // clamped into the grid.
 __device__ __forceinline__ int cell_of(float x, float k, int g) {
@@ -232,21 +223,36 @@ __device__ __forceinline__ int cell_of(float x, float k, int g) {
     return c < 0 ? 0 : (c >= g ? g - 1 : c);
 }
 
+// Cell keys, and the particle ids the sort carries as values.
 __global__ void k_cell_keys(const float *__restrict__ x, const float *__restrict__ y, int n,
-                            float kx, float ky, int gx, int gy, uint32_t *__restrict__ keys) {
+                            float kx, float ky, int gx, int gy, uint32_t *__restrict__ keys,
+                            uint32_t *__restrict__ ids) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     keys[i] = (uint32_t)(cell_of(y[i], ky, gy) * gx + cell_of(x[i], kx, gx));
+    ids[i] = (uint32_t)i;
 }
 
-__global__ void k_cell_bounds(const uint32_t *__restrict__ sk, int n, uint32_t ncells,
-                              uint32_t *__restrict__ start, uint32_t *__restrict__ end) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const uint32_t c = sk[p];
-    if (c >= ncells) return;  // keys are cells by construction: kept as a bound
-    if (p == 0 || sk[p - 1] != c) start[c] = (uint32_t)p;
-    if (p == n - 1 || sk[p + 1] != c) end[c] = (uint32_t)p + 1;
+// start[c] = the lower bound of cell c in the sorted keys (binary search;
+// the top levels of every search hit the same few cache lines).
+__global__ void k_cell_start(const uint32_t *__restrict__ sk, int n, uint32_t *__restrict__ start,
+                             uint32_t ncells) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    uint32_t lo = 0, hi = (uint32_t)n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (sk[mid] < c) lo = mid + 1;
+        else hi = mid;
+    }
+    start[c] = lo;
+}
+
+// end[c] = start[c + 1], and n for the last cell.
+__global__ void k_cell_end(const uint32_t *__restrict__ start, uint32_t *__restrict__ end,
+                           uint32_t ncells, int n) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < ncells) end[c] = c + 1 < ncells ? start[c + 1] : (uint32_t)n;
 }
 
 __global__ void k_gather_xy(const uint32_t *__restrict__ order, const float *__restrict__ x,
@@ -260,6 +266,20 @@ __global__ void k_gather_xy(const uint32_t *__restrict__ order, const float *__r
     ys[p] = y[i];
 }
 
+__global__ void k_pack_xy(const float *__restrict__ xs, const float *__restrict__ ys,
+                          float2 *__restrict__ xy, int n) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) xy[p] = make_float2(xs[p], ys[p]);
+}
+
+__global__ void k_gather_u32(const uint32_t *__restrict__ index, const uint32_t *__restrict__ in,
+                             uint32_t *__restrict__ out, int n) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t i = index[p];
+    out[p] = i < (uint32_t)n ? in[i] : 0u;
+}
+
 // Periodic displacement on an axis of length L (half = L/2).
 __device__ __forceinline__ float periodic(float d, float L, float half) {
     return d > half ? d - L : (d < -half ? d + L : d);
@@ -267,32 +287,65 @@ __device__ __forceinline__ float periodic(float d, float L, float half) {
 
 struct NArgs {
     const float *xs, *ys;
-    const uint32_t *order, *sk, *start, *end;
+    const float2 *xy;       // (xs, ys) interleaved, read by the candidate loops
+    const uint32_t *order;  // null: outputs indexed by sorted position
+    const uint32_t *sk, *start, *end;
     int n, gx, gy;
     float W, H, half_w, half_h, r2;
+    __device__ uint32_t id(uint32_t p) const { return order ? order[p] : p; }
 };
 
 // Visits every particle q != p of the 3x3 cells around p's cell (rows then
-// columns, -1..1, then q ascending) within radius r: f(q, d2).
+// columns, -1..1, then q ascending) within radius r: f(q, d2).  The cells of
+// one row are consecutive keys, so the three are one range of sorted
+// positions, [start of the left cell, end of the right cell), split in two
+// where the row wraps.
 template <typename F>
 __device__ __forceinline__ void for_neighbors(const NArgs &a, int p, F &&f) {
     const uint32_t c = a.sk[p];
     if (c >= (uint32_t)a.gx * (uint32_t)a.gy) return;  // a cell by construction: kept as a bound
     const int cx = (int)(c % (uint32_t)a.gx), cy = (int)(c / (uint32_t)a.gx);
-    const float px = a.xs[p], py = a.ys[p];
+    const float2 pp = a.xy[p];
+    const float px = pp.x, py = pp.y;
+    // Away from the domain's edges no candidate is more than 3 cells away,
+    // less than half the domain when it spans 7 or more cells: there the
+    // periodic correction is the identity and is skipped.
+    const bool interior = cx >= 1 && cx <= a.gx - 2 && cy >= 1 && cy <= a.gy - 2 && a.gx >= 7 &&
+                          a.gy >= 7;
+    // Candidates are taken four at a time, their loads issued together, and
+    // visited in ascending q as one at a time would.
+    auto visit = [&](uint32_t q, float2 o) {
+        float dx = o.x - px, dy = o.y - py;
+        if (!interior) {
+            dx = periodic(dx, a.W, a.half_w);
+            dy = periodic(dy, a.H, a.half_h);
+        }
+        const float d2 = dx * dx + dy * dy;
+        if (d2 < a.r2 && (int)q != p) f(q, d2);
+    };
+    auto range = [&](uint32_t q0, uint32_t q1) {
+        q1 = min(q1, (uint32_t)a.n);
+        uint32_t q = q0;
+        for (; q + 4 <= q1; q += 4) {
+            const float2 o0 = a.xy[q], o1 = a.xy[q + 1], o2 = a.xy[q + 2], o3 = a.xy[q + 3];
+            visit(q, o0);
+            visit(q + 1, o1);
+            visit(q + 2, o2);
+            visit(q + 3, o3);
+        }
+        for (; q < q1; ++q) visit(q, a.xy[q]);
+    };
     for (int oy = -1; oy <= 1; ++oy) {
         const int ny = cy + oy < 0 ? cy + oy + a.gy : (cy + oy >= a.gy ? cy + oy - a.gy : cy + oy);
-        for (int ox = -1; ox <= 1; ++ox) {
-            const int nx = cx + ox < 0 ? cx + ox + a.gx : (cx + ox >= a.gx ? cx + ox - a.gx : cx + ox);
-            const uint32_t nc = (uint32_t)(ny * a.gx + nx);
-            const uint32_t q1 = min(a.end[nc], (uint32_t)a.n);
-            for (uint32_t q = a.start[nc]; q < q1; ++q) {
-                if ((int)q == p) continue;
-                const float dx = periodic(a.xs[q] - px, a.W, a.half_w);
-                const float dy = periodic(a.ys[q] - py, a.H, a.half_h);
-                const float d2 = dx * dx + dy * dy;
-                if (d2 < a.r2) f(q, d2);
-            }
+        const uint32_t row = (uint32_t)(ny * a.gx);
+        if (cx == 0) {  // cells gx-1 | 0, 1
+            range(a.start[row + a.gx - 1], a.end[row + a.gx - 1]);
+            range(a.start[row], a.end[row + 1]);
+        } else if (cx == a.gx - 1) {  // cells gx-2, gx-1 | 0
+            range(a.start[row + cx - 1], a.end[row + cx]);
+            range(a.start[row], a.end[row]);
+        } else {
+            range(a.start[row + cx - 1], a.end[row + cx + 1]);
         }
     }
 }
@@ -302,7 +355,7 @@ __global__ void k_nbr_count(NArgs a, uint32_t *__restrict__ counts) {
     if (p >= a.n) return;
     uint32_t c = 0;
     for_neighbors(a, p, [&](uint32_t, float) { ++c; });
-    const uint32_t i = a.order[p];
+    const uint32_t i = a.id((uint32_t)p);
     if (i < (uint32_t)a.n) counts[i] = c;
 }
 
@@ -310,18 +363,20 @@ __global__ void k_nbr_fill(NArgs a, const uint32_t *__restrict__ offsets, uint32
                            size_t capacity) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.n) return;
-    const uint32_t i = a.order[p];
+    const uint32_t i = a.id((uint32_t)p);
     if (i >= (uint32_t)a.n) return;
     size_t o = offsets[i];
     for_neighbors(a, p, [&](uint32_t q, float) {
-        if (o < capacity) nbr[o] = a.order[q];
+        if (o < capacity) nbr[o] = a.id(q);
         ++o;
     });
 }
 
 // SPH density with the 2D poly6 kernel, W(d) = c (r^2 - d^2)^3, summed with
-// Kahan compensation in the traversal order, self term first.
-__global__ void k_density(NArgs a, float mass, float coef, float *__restrict__ rho) {
+// Kahan compensation in the traversal order, self term first; with counts,
+// the neighbour counts of the same traversal.
+__global__ void k_density(NArgs a, float mass, float coef, float *__restrict__ rho,
+                          uint32_t *__restrict__ counts) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.n) return;
     float sum = 0.0f, comp = 0.0f;
@@ -334,9 +389,16 @@ __global__ void k_density(NArgs a, float mass, float coef, float *__restrict__ r
         sum = s;
     };
     add(0.0f);
-    for_neighbors(a, p, [&](uint32_t, float d2) { add(d2); });
-    const uint32_t i = a.order[p];
-    if (i < (uint32_t)a.n) rho[i] = sum;
+    uint32_t cnt = 0;
+    for_neighbors(a, p, [&](uint32_t, float d2) {
+        add(d2);
+        ++cnt;
+    });
+    const uint32_t i = a.id((uint32_t)p);
+    if (i < (uint32_t)a.n) {
+        rho[i] = sum;
+        if (counts) counts[i] = cnt;
+    }
 }
 
 // A position wrapped into [0, n): unchanged when already inside (the
@@ -383,23 +445,31 @@ int passes_for(uint32_t ncells) {
 // digit histogram and its scan (kRadix * nblocks each), scan block sums.
 struct SortWs {
     uint32_t *k0, *k1, *v1, *hist, *hscan, *sums;
+    float2 *xy;  // interleaved sorted positions for the neighbour passes
 };
-size_t sort_ws_words(int n) {
+// Word offsets: k0 0, k1 n, v1 2n, hist 3n, hscan 3n + H, xy (float2,
+// rounded up to an even word) then sums.
+size_t hist_words(int n) {
     const size_t nb = (size_t)nblocks_for((size_t)n);
-    const size_t hist = (size_t)kRadix * (nb ? nb : 1);
-    return 3 * (size_t)n + 2 * hist + (size_t)nblocks_for(hist) + (size_t)nblocks_for((size_t)n) + 64;
+    return (size_t)kRadix * (nb ? nb : 1);
 }
-SortWs sort_ws(void *ws, int n) {
+size_t xy_offset(int n) { return (3 * (size_t)n + 2 * hist_words(n) + 1) & ~(size_t)1; }
+size_t sort_ws_words(int n, size_t ncells) {
+    const size_t hist = hist_words(n);
+    size_t sums = std::max<size_t>(nblocks_for(hist), nblocks_for((size_t)n));
+    sums = std::max<size_t>(sums, nblocks_for(ncells));
+    return xy_offset(n) + 2 * (size_t)n + sums + 64;
+}
+SortWs sort_ws(void *ws, int n) {  // ws: 8-byte aligned
     uint32_t *p = static_cast<uint32_t *>(ws);
-    const size_t nb = (size_t)nblocks_for((size_t)n);
-    const size_t hist = (size_t)kRadix * (nb ? nb : 1);
     SortWs s;
     s.k0 = p;
     s.k1 = p + n;
     s.v1 = p + 2 * (size_t)n;
     s.hist = p + 3 * (size_t)n;
-    s.hscan = s.hist + hist;
-    s.sums = s.hscan + hist;
+    s.hscan = s.hist + hist_words(n);
+    s.xy = reinterpret_cast<float2 *>(p + xy_offset(n));
+    s.sums = p + xy_offset(n) + 2 * (size_t)n;
     return s;
 }
 
@@ -409,11 +479,12 @@ bool valid_grid(const pfs_ngrid *g) {
            g->radius <= g->domain_w / g->gx && g->radius <= g->domain_h / g->gy;
 }
 
-NArgs nargs(const pfs_ngrid *g, const float *xs, const float *ys, const uint32_t *order,
-            const uint32_t *sk, const uint32_t *start, const uint32_t *end) {
+NArgs nargs(const pfs_ngrid *g, const float *xs, const float *ys, const float2 *xy,
+            const uint32_t *order, const uint32_t *sk, const uint32_t *start, const uint32_t *end) {
     NArgs a{};
     a.xs = xs;
     a.ys = ys;
+    a.xy = xy;
     a.order = order;
     a.sk = sk;
     a.start = start;
@@ -459,7 +530,7 @@ int pfs_ngrid_init(pfs_ngrid *g, int n, float domain_w, float domain_h, float ra
 
 size_t pfs_ngrid_workspace_bytes(const pfs_ngrid *g) {
     if (!g || g->n < 1) return 0;
-    return sort_ws_words(g->n) * sizeof(uint32_t);
+    return sort_ws_words(g->n, (size_t)g->gx * g->gy) * sizeof(uint32_t);
 }
 
 int pfs_scan_exclusive_u32(const uint32_t *in, uint32_t *out, int n, uint32_t *total, void *ws,
@@ -468,6 +539,15 @@ int pfs_scan_exclusive_u32(const uint32_t *in, uint32_t *out, int n, uint32_t *t
     if (!ws || ws_bytes < (size_t)(nblocks_for((size_t)n) + 1) * sizeof(uint32_t))
         return set_error(PFS_ERR_INVALID, "scan: workspace too small");
     return scan_exclusive(in, out, n, total, static_cast<uint32_t *>(ws), (hipStream_t)stream);
+}
+
+int pfs_gather_u32(const uint32_t *index, const uint32_t *in, uint32_t *out, int n, void *stream) {
+    if (n < 0 || (n > 0 && (!index || !in || !out)) || in == out)
+        return set_error(PFS_ERR_INVALID, "gather: bad arguments");
+    if (n == 0) return PFS_OK;
+    hipLaunchKernelGGL(k_gather_u32, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, index, in,
+                       out, n);
+    return check_launch("gather");
 }
 
 int pfs_particles_advect(float *x, float *y, int n, const pfs_field *vp, float dt, void *stream) {
@@ -487,14 +567,14 @@ int pfs_ngrid_build(const pfs_ngrid *g, const float *x, const float *y, uint32_t
     if (!valid_grid(g)) return set_error(PFS_ERR_INVALID, "ngrid: not initialised by pfs_ngrid_init");
     if (!x || !y || !keys || !order || !cell_start || !cell_end || !xs || !ys)
         return set_error(PFS_ERR_INVALID, "ngrid: NULL buffer");
-    if (!ws || ws_bytes < pfs_ngrid_workspace_bytes(g))
-        return set_error(PFS_ERR_INVALID, "ngrid: workspace smaller than pfs_ngrid_workspace_bytes()");
+    if (!ws || ws_bytes < pfs_ngrid_workspace_bytes(g) || ((uintptr_t)ws & 7))
+        return set_error(PFS_ERR_INVALID,
+                         "ngrid: workspace not 8-byte aligned or smaller than pfs_ngrid_workspace_bytes()");
     hipStream_t s = (hipStream_t)stream;
     const int n = g->n, nb = nblocks_for((size_t)n);
     const uint32_t ncells = (uint32_t)g->gx * (uint32_t)g->gy;
     SortWs w = sort_ws(ws, n);
-    hipLaunchKernelGGL(k_cell_keys, dim3(grid_for(n)), dim3(256), 0, s, x, y, n, g->kx, g->ky,
-                       g->gx, g->gy, keys);
+
     // Values start as particle ids in the buffer that makes the last pass
     // land in `order`.
     // Buffers alternate so that the last pass writes the keys to k0 (kept for
@@ -502,7 +582,8 @@ int pfs_ngrid_build(const pfs_ngrid *g, const float *x, const float *y, uint32_t
     const int passes = passes_for(ncells);
     uint32_t *kb[2] = {passes % 2 == 0 ? w.k0 : w.k1, passes % 2 == 0 ? w.k1 : w.k0};
     uint32_t *vb[2] = {passes % 2 == 0 ? order : w.v1, passes % 2 == 0 ? w.v1 : order};
-    hipLaunchKernelGGL(k_iota, dim3(grid_for(n)), dim3(256), 0, s, vb[0], n);
+    hipLaunchKernelGGL(k_cell_keys, dim3(grid_for(n)), dim3(256), 0, s, x, y, n, g->kx, g->ky,
+                       g->gx, g->gy, keys, vb[0]);
     const uint32_t *kin = keys;
     for (int p = 0; p < passes; ++p) {
         const int shift = 8 * p;
@@ -512,12 +593,11 @@ int pfs_ngrid_build(const pfs_ngrid *g, const float *x, const float *y, uint32_t
                            kb[(p + 1) & 1], vb[(p + 1) & 1], n, shift, w.hscan, nb);
         kin = kb[(p + 1) & 1];
     }
-    const uint32_t *sk = kin;
-    const int fill_blocks = (int)std::min<size_t>(((size_t)ncells + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_fill_u32, dim3(fill_blocks), dim3(256), 0, s, cell_start, (size_t)ncells, 0u);
-    hipLaunchKernelGGL(k_fill_u32, dim3(fill_blocks), dim3(256), 0, s, cell_end, (size_t)ncells, 0u);
-    hipLaunchKernelGGL(k_cell_bounds, dim3(grid_for(n)), dim3(256), 0, s, sk, n, ncells, cell_start,
-                       cell_end);
+    // Cell tables: the lower bound of every cell (empty ones included) in the
+    // sorted keys, left in k0 by the last pass.
+    const unsigned cblocks = (unsigned)(((size_t)ncells + 255) / 256);
+    hipLaunchKernelGGL(k_cell_start, dim3(cblocks), dim3(256), 0, s, w.k0, n, cell_start, ncells);
+    hipLaunchKernelGGL(k_cell_end, dim3(cblocks), dim3(256), 0, s, cell_start, cell_end, ncells, n);
     hipLaunchKernelGGL(k_gather_xy, dim3(grid_for(n)), dim3(256), 0, s, order, x, y, xs, ys, n);
     return check_launch("neighbour grid");
 }
@@ -527,13 +607,15 @@ int pfs_ngrid_neighbors(const pfs_ngrid *g, const float *xs, const float *ys, co
                         uint32_t *offsets, uint32_t *nbr, size_t nbr_capacity, uint32_t *total,
                         void *ws, size_t ws_bytes, void *stream) {
     if (!valid_grid(g)) return set_error(PFS_ERR_INVALID, "ngrid: not initialised by pfs_ngrid_init");
-    if (!xs || !ys || !order || !cell_start || !cell_end || !counts || !offsets || !total)
+    if (!xs || !ys || !cell_start || !cell_end || !counts || !offsets || !total)
         return set_error(PFS_ERR_INVALID, "ngrid: NULL buffer");
-    if (!ws || ws_bytes < pfs_ngrid_workspace_bytes(g))
-        return set_error(PFS_ERR_INVALID, "ngrid: workspace smaller than pfs_ngrid_workspace_bytes()");
+    if (!ws || ws_bytes < pfs_ngrid_workspace_bytes(g) || ((uintptr_t)ws & 7))
+        return set_error(PFS_ERR_INVALID,
+                         "ngrid: workspace not 8-byte aligned or smaller than pfs_ngrid_workspace_bytes()");
     hipStream_t s = (hipStream_t)stream;
     SortWs w = sort_ws(ws, g->n);
-    const NArgs a = nargs(g, xs, ys, order, w.k0, cell_start, cell_end);
+    const NArgs a = nargs(g, xs, ys, w.xy, order, w.k0, cell_start, cell_end);
+    hipLaunchKernelGGL(k_pack_xy, dim3(grid_for(g->n)), dim3(256), 0, s, xs, ys, w.xy, g->n);
     hipLaunchKernelGGL(k_nbr_count, dim3(grid_for(g->n)), dim3(256), 0, s, a, counts);
     PFS_TRY(scan_exclusive(counts, offsets, g->n, total, w.sums, s));
     if (!nbr) return PFS_OK;  // counts and offsets only
@@ -551,19 +633,21 @@ int pfs_ngrid_neighbors(const pfs_ngrid *g, const float *xs, const float *ys, co
 
 int pfs_ngrid_density(const pfs_ngrid *g, const float *xs, const float *ys, const uint32_t *order,
                       const uint32_t *cell_start, const uint32_t *cell_end, float mass, float *rho,
-                      void *ws, size_t ws_bytes, void *stream) {
+                      uint32_t *counts, void *ws, size_t ws_bytes, void *stream) {
     if (!valid_grid(g)) return set_error(PFS_ERR_INVALID, "ngrid: not initialised by pfs_ngrid_init");
-    if (!xs || !ys || !order || !cell_start || !cell_end || !rho)
+    if (!xs || !ys || !cell_start || !cell_end || !rho)
         return set_error(PFS_ERR_INVALID, "ngrid: NULL buffer");
-    if (!ws || ws_bytes < pfs_ngrid_workspace_bytes(g))
-        return set_error(PFS_ERR_INVALID, "ngrid: workspace smaller than pfs_ngrid_workspace_bytes()");
+    if (!ws || ws_bytes < pfs_ngrid_workspace_bytes(g) || ((uintptr_t)ws & 7))
+        return set_error(PFS_ERR_INVALID,
+                         "ngrid: workspace not 8-byte aligned or smaller than pfs_ngrid_workspace_bytes()");
     SortWs w = sort_ws(ws, g->n);
-    const NArgs a = nargs(g, xs, ys, order, w.k0, cell_start, cell_end);
+    hipStream_t s = (hipStream_t)stream;
+    const NArgs a = nargs(g, xs, ys, w.xy, order, w.k0, cell_start, cell_end);
+    hipLaunchKernelGGL(k_pack_xy, dim3(grid_for(g->n)), dim3(256), 0, s, xs, ys, w.xy, g->n);
     // 2D poly6: 4 / (pi r^8), in float from the double constant.
     const double r = g->radius;
     const float coef = (float)(4.0 / (3.14159265358979323846 * r * r * r * r * r * r * r * r));
-    hipLaunchKernelGGL(k_density, dim3(grid_for(g->n)), dim3(256), 0, (hipStream_t)stream, a, mass,
-                       coef, rho);
+    hipLaunchKernelGGL(k_density, dim3(grid_for(g->n)), dim3(256), 0, s, a, mass, coef, rho, counts);
     return check_launch("density");
 }
 

@@ -88,31 +88,87 @@ class NeighborGrid:
                   _ptr(self.order), _ptr(self.cell_start), _ptr(self.cell_end), _ptr(self.xs),
                   _ptr(self.ys), _ptr(self.ws), self.ws.numel() * 4, _stream(stream))
 
-    def _common(self):
-        return (ctypes.byref(self.g), _ptr(self.xs), _ptr(self.ys), _ptr(self.order),
+    def _common(self, xs=None, ys=None, by_position=False):
+        xs = self.xs if xs is None else xs
+        ys = self.ys if ys is None else ys
+        return (ctypes.byref(self.g), _ptr(xs), _ptr(ys), None if by_position else _ptr(self.order),
                 _ptr(self.cell_start), _ptr(self.cell_end))
 
-    def neighbor_counts(self, stream=None) -> Tuple[torch.Tensor, torch.Tensor]:
-        """(counts, exclusive offsets) by particle id, without the lists."""
-        _lib.call("pfs_ngrid_neighbors", *self._common(), _ptr(self.counts), _ptr(self.offsets),
-                  None, 0, _ptr(self.total), _ptr(self.ws), self.ws.numel() * 4, _stream(stream))
+    def neighbor_counts(self, stream=None, xs=None, ys=None, by_position=False):
+        """(counts, exclusive offsets) by particle id (by sorted position with
+        by_position), without the lists."""
+        _lib.call("pfs_ngrid_neighbors", *self._common(xs, ys, by_position), _ptr(self.counts),
+                  _ptr(self.offsets), None, 0, _ptr(self.total), _ptr(self.ws),
+                  self.ws.numel() * 4, _stream(stream))
         return self.counts, self.offsets
 
-    def neighbors(self, stream=None) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+    def neighbors(self, stream=None, xs=None, ys=None, by_position=False):
         """(counts, offsets, lists): particle i's neighbours are
         lists[offsets[i] : offsets[i] + counts[i]].  Synchronises the stream
         to size the lists."""
-        self.neighbor_counts(stream)
+        self.neighbor_counts(stream, xs, ys, by_position)
         total = int(self.total.item()) & 0xFFFFFFFF
         nbr = torch.empty(max(total, 1), dtype=torch.int32, device=self.keys.device)
-        _lib.call("pfs_ngrid_neighbors", *self._common(), _ptr(self.counts), _ptr(self.offsets),
-                  _ptr(nbr), nbr.numel(), _ptr(self.total), _ptr(self.ws), self.ws.numel() * 4,
-                  _stream(stream))
+        _lib.call("pfs_ngrid_neighbors", *self._common(xs, ys, by_position), _ptr(self.counts),
+                  _ptr(self.offsets), _ptr(nbr), nbr.numel(), _ptr(self.total), _ptr(self.ws),
+                  self.ws.numel() * 4, _stream(stream))
         return self.counts, self.offsets, nbr[:total]
 
-    def density(self, mass: float, stream=None) -> torch.Tensor:
-        """SPH (2D poly6) density per particle id."""
-        rho = torch.empty(self.n, dtype=torch.float32, device=self.keys.device)
-        _lib.call("pfs_ngrid_density", *self._common(), mass, _ptr(rho), _ptr(self.ws),
-                  self.ws.numel() * 4, _stream(stream))
-        return rho
+    def density(self, mass: float, stream=None, xs=None, ys=None, by_position=False,
+                out: torch.Tensor = None, with_counts: bool = False):
+        """SPH (2D poly6) density per particle id (per sorted position with
+        by_position); with_counts: (density, neighbour counts) from the same
+        traversal."""
+        rho = out if out is not None else torch.empty(self.n, dtype=torch.float32,
+                                                      device=self.keys.device)
+        _lib.call("pfs_ngrid_density", *self._common(xs, ys, by_position), mass, _ptr(rho),
+                  _ptr(self.counts) if with_counts else None, _ptr(self.ws), self.ws.numel() * 4,
+                  _stream(stream))
+        return (rho, self.counts) if with_counts else rho
+
+
+def gather(index: torch.Tensor, src: torch.Tensor, out: torch.Tensor, stream=None) -> None:
+    """out[p] = src[index[p]] for int32 tensors (uint32 in the C ABI)."""
+    for t in (index, src, out):
+        if t.dtype != torch.int32 or not t.is_contiguous():
+            raise ValueError("need contiguous int32 tensors")
+    _lib.call("pfs_gather_u32", _ptr(index), _ptr(src), _ptr(out), index.numel(),
+              _stream(stream))
+
+
+class Tracers:
+    """Tracers kept in cell order.  Every sort() rebuilds the neighbour grid
+    and adopts its sorted order: positions become the sorted positions and
+    `ids` (the original particle numbers) are permuted alongside, so gathers
+    from the velocity field, the next sort's input and the neighbour passes
+    all touch memory in cell order.  Neighbour lists and densities are then
+    indexed by the current (sorted) position."""
+
+    def __init__(self, x: torch.Tensor, y: torch.Tensor, domain_w: float, domain_h: float,
+                 radius: float):
+        _check_xy(x, y)
+        self.x, self.y = x.clone(), y.clone()
+        self.ids = torch.arange(x.numel(), dtype=torch.int32, device=x.device)
+        self._ids = torch.empty_like(self.ids)
+        self.grid = NeighborGrid(x.numel(), domain_w, domain_h, radius, device=str(x.device))
+
+    def advect(self, vp: VPField, dt: float, stream=None) -> None:
+        advect(self.x, self.y, vp, dt, stream)
+
+    def sort(self, stream=None) -> None:
+        g = self.grid
+        g.build(self.x, self.y, stream)
+        self.x, g.xs = g.xs, self.x
+        self.y, g.ys = g.ys, self.y
+        gather(g.order, self.ids, self._ids, stream)
+        self.ids, self._ids = self._ids, self.ids
+
+    def neighbor_counts(self, stream=None):
+        return self.grid.neighbor_counts(stream, self.x, self.y, by_position=True)
+
+    def neighbors(self, stream=None):
+        return self.grid.neighbors(stream, self.x, self.y, by_position=True)
+
+    def density(self, mass: float, stream=None, out=None, with_counts: bool = False):
+        return self.grid.density(mass, stream, self.x, self.y, by_position=True, out=out,
+                                 with_counts=with_counts)

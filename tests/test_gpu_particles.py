@@ -56,13 +56,16 @@ def _check_grid(x, y, W, H, r, mass=1.0):
     np.testing.assert_array_equal(_u32(nbr), on)
     rho = ng.density(mass)
     assert rho.cpu().numpy().tobytes() == og.density(mass).tobytes()
+    rho2, c2 = ng.density(mass, with_counts=True)
+    assert rho2.cpu().numpy().tobytes() == og.density(mass).tobytes()
+    np.testing.assert_array_equal(_u32(c2), oc)
     return ng
 
 
 # cell counts spanning one (< 256 cells), two and three radix passes
 @pytest.mark.parametrize("n,W,H,r", [(1, 16.0, 16.0, 1.0), (777, 40.0, 30.0, 2.5),
                                      (5000, 100.0, 100.0, 1.5), (9000, 500.0, 300.0, 1.0),
-                                     (4097, 64.0, 64.0, 3.9)])
+                                     (4097, 64.0, 64.0, 3.9), (600, 20.0, 13.0, 3.9)])
 def test_grid_and_neighbors_match_oracle(n, W, H, r):
     x, y = _positions(n, W, H, seed=n)
     _check_grid(x, y, W, H, r)
@@ -116,3 +119,30 @@ def test_errors():
     x = torch.zeros(11, device="cuda")
     with pytest.raises(ValueError):
         ng.build(x, x)
+
+
+def test_tracers_kept_in_cell_order_match_oracle():
+    """Tracers that adopt the sort every step: positions, ids, tables, lists
+    (by sorted position) and densities against the oracle doing the same."""
+    s = scenes.random_scene(96, 80, 96, 80, seed=21, dt=0.8, visc=0.1, iters=4)
+    x, y = scenes.particles_disk(6000, 96, 80, seed=21)
+    vp = VPField.from_numpy(s.vp)
+    tr = particles.Tracers(torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda(), 96.0, 80.0, 1.5)
+    g = tr.grid.g
+    xo, yo, ids = x.copy(), y.copy(), np.arange(len(x), dtype=np.uint32)
+    for step in range(4):
+        tr.advect(vp, s.dt)
+        tr.sort()
+        op.advect(xo, yo, s.vp, s.dt)
+        og = op.Grid(xo, yo, 96.0, 80.0, 1.5, g.gx, g.gy, g.kx, g.ky)
+        xo, yo, ids = og.xs.copy(), og.ys.copy(), ids[og.order]
+        assert tr.x.cpu().numpy().tobytes() == xo.tobytes(), step
+        np.testing.assert_array_equal(_u32(tr.ids), ids)
+        np.testing.assert_array_equal(_u32(tr.grid.order), og.order)
+        np.testing.assert_array_equal(_u32(tr.grid.cell_start), og.start)
+    counts, offsets, nbr = tr.neighbors()
+    oc, oo, on = og.neighbors(by_position=True)
+    np.testing.assert_array_equal(_u32(counts), oc)
+    np.testing.assert_array_equal(_u32(offsets), oo)
+    np.testing.assert_array_equal(_u32(nbr), on)
+    assert tr.density(0.5).cpu().numpy().tobytes() == og.density(0.5, by_position=True).tobytes()

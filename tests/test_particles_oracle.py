@@ -55,10 +55,8 @@ def test_sort_and_tables_against_numpy(n, W, H, r, seed):
     np.testing.assert_array_equal(og.order, order)
     sk = keys[order]
     cells = np.arange(g.gx * g.gy)
-    lo, hi = np.searchsorted(sk, cells, "left"), np.searchsorted(sk, cells, "right")
-    empty = lo == hi
-    np.testing.assert_array_equal(og.start, np.where(empty, 0, lo))
-    np.testing.assert_array_equal(og.end, np.where(empty, 0, hi))
+    np.testing.assert_array_equal(og.start, np.searchsorted(sk, cells, "left"))
+    np.testing.assert_array_equal(og.end, np.searchsorted(sk, cells, "right"))
     np.testing.assert_array_equal(og.xs, x[order])
     np.testing.assert_array_equal(og.ys, y[order])
 
