@@ -8,7 +8,7 @@
 //                            corner keys and blend order (:55-66); positions
 //                            wrap back into [0, W) exactly;
 //   pfs_ngrid_build          cell keys; a stable LSD radix sort of
-//                            (key, particle) pairs, 8-bit digits: per-block
+//                            (key, particle) pairs, 8- or 10-bit digits: per-block
 //                            LDS histograms, a DPP prefix scan, and a scatter
 //                            whose in-block ranks come from wavefront ballots;
 //                            cell start/end tables (the scan of the cell
@@ -41,7 +41,7 @@ constexpr int kThreads = 1024;              // sort, scan and grid kernels
 constexpr int kItems = 4;                   // keys per thread
 constexpr int kTile = kThreads * kItems;    // keys per block
 constexpr int kWaves = kThreads / 64;
-constexpr int kRadix = 256;                 // 8-bit digits
+constexpr int kMaxRadixBits = 10;           // digits of 8 or 10 bits
 
 // ---- wavefront and block scans ------------------------------------------
 // Inclusive scan across the 64 lanes: four row shifts (1, 2, 4, 8) give
@@ -142,49 +142,56 @@ int scan_exclusive(const uint32_t *in, uint32_t *out, int n, uint32_t *grand, ui
 }
 
 // ---- stable LSD radix sort of (key, value) pairs ---------------------------
+// Digits of RB bits (8 or 10).  The histogram of block b is stored digit
+// major, hist[d * nb + b], so that its exclusive scan is every (digit,
+// block)'s first output position.
+template <int RB>
 __global__ __launch_bounds__(kThreads) void k_radix_hist(const uint32_t *__restrict__ keys, int n,
                                                          int shift, uint32_t *__restrict__ hist,
                                                          int nb) {
-    __shared__ uint32_t h[kRadix];
-    if (threadIdx.x < kRadix) h[threadIdx.x] = 0;
+    constexpr int R = 1 << RB;
+    __shared__ uint32_t h[R];
+    for (int d = threadIdx.x; d < R; d += kThreads) h[d] = 0;
     __syncthreads();
     const size_t base = (size_t)blockIdx.x * kTile;
 #pragma unroll
     for (int r = 0; r < kItems; ++r) {
         const size_t i = base + (size_t)r * kThreads + threadIdx.x;
-        if (i < (size_t)n) atomicAdd(&h[(keys[i] >> shift) & (kRadix - 1)], 1u);
+        if (i < (size_t)n) atomicAdd(&h[(keys[i] >> shift) & (R - 1)], 1u);
     }
     __syncthreads();
-    if (threadIdx.x < kRadix) hist[(size_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+    for (int d = threadIdx.x; d < R; d += kThreads) hist[(size_t)d * nb + blockIdx.x] = h[d];
 }
 
 // Scatter of one pass.  Items are taken in tile order (round r covers
 // [r*kThreads, (r+1)*kThreads) of the block's tile), so the rank of an item
 // among equal digits is: the digit's offset for this block (the scan of the
 // digit-major histogram), plus the equal digits of earlier rounds and earlier
-// waves (LDS counts), plus the equal-digit lanes below it in its wave
-// (eight ballots, one per digit bit, intersected).
+// waves (LDS counts), plus the equal-digit lanes below it in its wave (RB
+// ballots, one per digit bit, intersected).
+template <int RB>
 __global__ __launch_bounds__(kThreads) void k_radix_scatter(
     const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin, uint32_t *__restrict__ kout,
     uint32_t *__restrict__ vout, int n, int shift, const uint32_t *__restrict__ offsets, int nb) {
-    __shared__ uint32_t wcount[kWaves][kRadix];
-    __shared__ uint32_t run[kRadix], gofs[kRadix];
+    constexpr int R = 1 << RB;
+    __shared__ uint32_t wcount[kWaves][R];
+    __shared__ uint32_t run[R], gofs[R];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (threadIdx.x < kRadix) {
-        gofs[threadIdx.x] = offsets[(size_t)threadIdx.x * nb + blockIdx.x];
-        run[threadIdx.x] = 0;
+    for (int d = threadIdx.x; d < R; d += kThreads) {
+        gofs[d] = offsets[(size_t)d * nb + blockIdx.x];
+        run[d] = 0;
     }
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const size_t base = (size_t)blockIdx.x * kTile;
     for (int r = 0; r < kItems; ++r) {
-        for (int e = threadIdx.x; e < kWaves * kRadix; e += kThreads) (&wcount[0][0])[e] = 0;
+        for (int e = threadIdx.x; e < kWaves * R; e += kThreads) (&wcount[0][0])[e] = 0;
         const size_t i = base + (size_t)r * kThreads + threadIdx.x;
         const bool valid = i < (size_t)n;
         const uint32_t key = valid ? kin[i] : 0u, val = valid ? vin[i] : 0u;
-        const uint32_t d = (key >> shift) & (kRadix - 1);
+        const uint32_t d = (key >> shift) & (R - 1);
         uint64_t peers = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < RB; ++b) {
             const bool bit = (d >> b) & 1u;
             const uint64_t m = __ballot(valid && bit);
             peers &= bit ? m : ~m;
@@ -194,14 +201,14 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
         __syncthreads();  // wcount cleared; previous round's reads done
         if (valid && rank == 0) wcount[wave][d] = (uint32_t)__popcll(peers);
         __syncthreads();
-        if (threadIdx.x < kRadix) {
-            uint32_t s = run[threadIdx.x];
+        for (int dd = threadIdx.x; dd < R; dd += kThreads) {
+            uint32_t s = run[dd];
             for (int w = 0; w < kWaves; ++w) {
-                const uint32_t c = wcount[w][threadIdx.x];
-                wcount[w][threadIdx.x] = s;
+                const uint32_t c = wcount[w][dd];
+                wcount[w][dd] = s;
                 s += c;
             }
-            run[threadIdx.x] = s;
+            run[dd] = s;
         }
         __syncthreads();
         if (valid) {
@@ -434,15 +441,23 @@ __global__ void k_tracer_advect(float *__restrict__ x, float *__restrict__ y, in
 
 int grid_for(int n) { return (n + 255) / 256; }
 
-int passes_for(uint32_t ncells) {
-    int bits = 0;
-    while (bits < 32 && (ncells - 1) >> bits) ++bits;
-    const int p = (bits + 7) / 8;
-    return p < 1 ? 1 : p;
+// The digit plan for keys below ncells: the fewest passes, 8-bit digits
+// where they suffice for that many passes, else 10-bit.
+struct RadixPlan {
+    int passes, bits;
+};
+RadixPlan radix_plan(uint32_t ncells) {
+    int kb = 0;
+    while (kb < 32 && (ncells - 1) >> kb) ++kb;
+    if (kb <= 8) return {1, 8};
+    if (kb <= 16) return {2, 8};
+    if (kb <= 20) return {2, 10};
+    if (kb <= 24) return {3, 8};
+    return {3, 10};
 }
 
 // Workspace layout (words): sorted keys (2 buffers), values spare buffer,
-// digit histogram and its scan (kRadix * nblocks each), scan block sums.
+// digit histogram and its scan (2^10 * nblocks each), scan block sums.
 struct SortWs {
     uint32_t *k0, *k1, *v1, *hist, *hscan, *sums;
     float2 *xy;  // interleaved sorted positions for the neighbour passes
@@ -451,7 +466,7 @@ struct SortWs {
 // rounded up to an even word) then sums.
 size_t hist_words(int n) {
     const size_t nb = (size_t)nblocks_for((size_t)n);
-    return (size_t)kRadix * (nb ? nb : 1);
+    return ((size_t)1 << kMaxRadixBits) * (nb ? nb : 1);
 }
 size_t xy_offset(int n) { return (3 * (size_t)n + 2 * hist_words(n) + 1) & ~(size_t)1; }
 size_t sort_ws_words(int n, size_t ncells) {
@@ -579,18 +594,28 @@ int pfs_ngrid_build(const pfs_ngrid *g, const float *x, const float *y, uint32_t
     // land in `order`.
     // Buffers alternate so that the last pass writes the keys to k0 (kept for
     // the neighbour passes) and the particle ids to `order`.
-    const int passes = passes_for(ncells);
+    const RadixPlan plan = radix_plan(ncells);
+    const int passes = plan.passes;
     uint32_t *kb[2] = {passes % 2 == 0 ? w.k0 : w.k1, passes % 2 == 0 ? w.k1 : w.k0};
     uint32_t *vb[2] = {passes % 2 == 0 ? order : w.v1, passes % 2 == 0 ? w.v1 : order};
     hipLaunchKernelGGL(k_cell_keys, dim3(grid_for(n)), dim3(256), 0, s, x, y, n, g->kx, g->ky,
                        g->gx, g->gy, keys, vb[0]);
     const uint32_t *kin = keys;
     for (int p = 0; p < passes; ++p) {
-        const int shift = 8 * p;
-        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kThreads), 0, s, kin, n, shift, w.hist, nb);
-        PFS_TRY(scan_exclusive(w.hist, w.hscan, kRadix * nb, nullptr, w.sums, s));
-        hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kThreads), 0, s, kin, vb[p & 1],
-                           kb[(p + 1) & 1], vb[(p + 1) & 1], n, shift, w.hscan, nb);
+        const int shift = plan.bits * p;
+        if (plan.bits == 8) {
+            hipLaunchKernelGGL(k_radix_hist<8>, dim3(nb), dim3(kThreads), 0, s, kin, n, shift, w.hist,
+                               nb);
+            PFS_TRY(scan_exclusive(w.hist, w.hscan, (1 << 8) * nb, nullptr, w.sums, s));
+            hipLaunchKernelGGL(k_radix_scatter<8>, dim3(nb), dim3(kThreads), 0, s, kin, vb[p & 1],
+                               kb[(p + 1) & 1], vb[(p + 1) & 1], n, shift, w.hscan, nb);
+        } else {
+            hipLaunchKernelGGL(k_radix_hist<10>, dim3(nb), dim3(kThreads), 0, s, kin, n, shift,
+                               w.hist, nb);
+            PFS_TRY(scan_exclusive(w.hist, w.hscan, (1 << 10) * nb, nullptr, w.sums, s));
+            hipLaunchKernelGGL(k_radix_scatter<10>, dim3(nb), dim3(kThreads), 0, s, kin, vb[p & 1],
+                               kb[(p + 1) & 1], vb[(p + 1) & 1], n, shift, w.hscan, nb);
+        }
         kin = kb[(p + 1) & 1];
     }
     // Cell tables: the lower bound of every cell (empty ones included) in the

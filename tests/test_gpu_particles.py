@@ -62,10 +62,12 @@ def _check_grid(x, y, W, H, r, mass=1.0):
     return ng
 
 
-# cell counts spanning one (< 256 cells), two and three radix passes
+# cell counts spanning every radix plan: 1 pass (< 2^8 cells), 2 x 8 bits,
+# 2 x 10 bits (150K cells), 3 x 8 bits (1.2M cells), 3 x 10 bits (17.6M cells)
 @pytest.mark.parametrize("n,W,H,r", [(1, 16.0, 16.0, 1.0), (777, 40.0, 30.0, 2.5),
                                      (5000, 100.0, 100.0, 1.5), (9000, 500.0, 300.0, 1.0),
-                                     (4097, 64.0, 64.0, 3.9), (600, 20.0, 13.0, 3.9)])
+                                     (4097, 64.0, 64.0, 3.9), (600, 20.0, 13.0, 3.9),
+                                     (3000, 1100.0, 1100.0, 1.0), (3000, 4200.0, 4200.0, 1.0)])
 def test_grid_and_neighbors_match_oracle(n, W, H, r):
     x, y = _positions(n, W, H, seed=n)
     _check_grid(x, y, W, H, r)
