@@ -91,26 +91,17 @@ __global__ __launch_bounds__(kThreads) void k_scan_reduce(const uint32_t *__rest
     if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
-// One block: exclusive scan of sums[0, nb) in place, in chunks of kThreads.
-__global__ __launch_bounds__(kThreads) void k_scan_top(uint32_t *__restrict__ sums, int nb,
-                                                       uint32_t *__restrict__ grand) {
-    __shared__ uint32_t sh[kWaves + 1];
-    uint32_t carry = 0;
-    for (int base = 0; base < nb; base += kThreads) {
-        const int i = base + (int)threadIdx.x;
-        const uint32_t v = i < nb ? sums[i] : 0u;
-        uint32_t total;
-        const uint32_t ex = block_excl_scan(v, sh, &total);
-        if (i < nb) sums[i] = carry + ex;
-        carry += total;
-    }
-    if (threadIdx.x == 0 && grand) *grand = carry;
-}
-
+// Block b adds up the totals of blocks [0, b) itself (a few hundred at
+// most), then scans its tile; the last block also writes the grand total.
 __global__ __launch_bounds__(kThreads) void k_scan_down(const uint32_t *__restrict__ in,
                                                         uint32_t *__restrict__ out, int n,
-                                                        const uint32_t *__restrict__ sums) {
+                                                        const uint32_t *__restrict__ sums,
+                                                        uint32_t *__restrict__ grand) {
     __shared__ uint32_t sh[kWaves + 1];
+    uint32_t before = 0;
+    for (int j = threadIdx.x; j < (int)blockIdx.x; j += kThreads) before += sums[j];
+    uint32_t offset;
+    (void)block_excl_scan(before, sh, &offset);
     const size_t base = (size_t)blockIdx.x * kTile + (size_t)threadIdx.x * kItems;
     uint32_t v[kItems], s = 0;
 #pragma unroll
@@ -119,25 +110,26 @@ __global__ __launch_bounds__(kThreads) void k_scan_down(const uint32_t *__restri
         s += v[k];
     }
     uint32_t total;
-    uint32_t run = sums[blockIdx.x] + block_excl_scan(s, sh, &total);
+    uint32_t run = offset + block_excl_scan(s, sh, &total);
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         if (base + k < (size_t)n) out[base + k] = run;
         run += v[k];
     }
+    if (grand && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *grand = offset + total;
 }
 
 int nblocks_for(size_t n) { return (int)((n + kTile - 1) / kTile); }
 
-// Exclusive scan of n values; grand (device, may be null) gets the total.
-// ws: nblocks_for(n) words.
+// Exclusive scan of n values in two kernels (block totals, then blocks
+// scanned from the totals before them); grand (device, may be null) gets the
+// total.  ws: nblocks_for(n) words.
 int scan_exclusive(const uint32_t *in, uint32_t *out, int n, uint32_t *grand, uint32_t *ws,
                    hipStream_t s) {
     const int nb = nblocks_for((size_t)n);
     if (nb == 0) return PFS_OK;
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kThreads), 0, s, in, n, ws);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kThreads), 0, s, ws, nb, grand);
-    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kThreads), 0, s, in, out, n, ws);
+    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kThreads), 0, s, in, out, n, ws, grand);
     return check_launch("exclusive scan");
 }
 
@@ -241,9 +233,10 @@ __global__ void k_cell_keys(const float *__restrict__ x, const float *__restrict
 }
 
 // start[c] = the lower bound of cell c in the sorted keys (binary search;
-// the top levels of every search hit the same few cache lines).
-__global__ void k_cell_start(const uint32_t *__restrict__ sk, int n, uint32_t *__restrict__ start,
-                             uint32_t ncells) {
+// the top levels of every search hit the same few cache lines), which is
+// also end[c - 1]; the last cell ends at n.
+__global__ void k_cell_tables(const uint32_t *__restrict__ sk, int n, uint32_t *__restrict__ start,
+                              uint32_t *__restrict__ end, uint32_t ncells) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncells) return;
     uint32_t lo = 0, hi = (uint32_t)n;
@@ -253,13 +246,8 @@ __global__ void k_cell_start(const uint32_t *__restrict__ sk, int n, uint32_t *_
         else hi = mid;
     }
     start[c] = lo;
-}
-
-// end[c] = start[c + 1], and n for the last cell.
-__global__ void k_cell_end(const uint32_t *__restrict__ start, uint32_t *__restrict__ end,
-                           uint32_t ncells, int n) {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < ncells) end[c] = c + 1 < ncells ? start[c + 1] : (uint32_t)n;
+    if (c > 0) end[c - 1] = lo;
+    if (c + 1 == ncells) end[c] = (uint32_t)n;
 }
 
 __global__ void k_gather_xy(const uint32_t *__restrict__ order, const float *__restrict__ x,
@@ -621,8 +609,8 @@ int pfs_ngrid_build(const pfs_ngrid *g, const float *x, const float *y, uint32_t
     // Cell tables: the lower bound of every cell (empty ones included) in the
     // sorted keys, left in k0 by the last pass.
     const unsigned cblocks = (unsigned)(((size_t)ncells + 255) / 256);
-    hipLaunchKernelGGL(k_cell_start, dim3(cblocks), dim3(256), 0, s, w.k0, n, cell_start, ncells);
-    hipLaunchKernelGGL(k_cell_end, dim3(cblocks), dim3(256), 0, s, cell_start, cell_end, ncells, n);
+    hipLaunchKernelGGL(k_cell_tables, dim3(cblocks), dim3(256), 0, s, w.k0, n, cell_start, cell_end,
+                       ncells);
     hipLaunchKernelGGL(k_gather_xy, dim3(grid_for(n)), dim3(256), 0, s, order, x, y, xs, ys, n);
     return check_launch("neighbour grid");
 }
