@@ -290,8 +290,9 @@ struct NArgs {
     __device__ uint32_t id(uint32_t p) const { return order ? order[p] : p; }
 };
 
-// Visits every particle q != p of the 3x3 cells around p's cell (rows then
-// columns, -1..1, then q ascending) within radius r: f(q, d2).  The cells of
+// Visits every particle q of the 3x3 cells around p's cell (rows then
+// columns, -1..1, then q ascending): f(q, d2, ok) with ok = q != p and
+// d2 < r^2, so that callers can accumulate without branching.  The cells of
 // one row are consecutive keys, so the three are one range of sorted
 // positions, [start of the left cell, end of the right cell), split in two
 // where the row wraps.
@@ -302,46 +303,56 @@ __device__ __forceinline__ void for_neighbors(const NArgs &a, int p, F &&f) {
     const int cx = (int)(c % (uint32_t)a.gx), cy = (int)(c / (uint32_t)a.gx);
     const float2 pp = a.xy[p];
     const float px = pp.x, py = pp.y;
-    // Away from the domain's edges no candidate is more than 3 cells away,
-    // less than half the domain when it spans 7 or more cells: there the
-    // periodic correction is the identity and is skipped.
-    const bool interior = cx >= 1 && cx <= a.gx - 2 && cy >= 1 && cy <= a.gy - 2 && a.gx >= 7 &&
-                          a.gy >= 7;
     // Candidates are taken four at a time, their loads issued together, and
-    // visited in ascending q as one at a time would.
-    auto visit = [&](uint32_t q, float2 o) {
-        float dx = o.x - px, dy = o.y - py;
-        if (!interior) {
-            dx = periodic(dx, a.W, a.half_w);
-            dy = periodic(dy, a.H, a.half_h);
-        }
-        const float d2 = dx * dx + dy * dy;
-        if (d2 < a.r2 && (int)q != p) f(q, d2);
-    };
-    auto range = [&](uint32_t q0, uint32_t q1) {
+    // visited in ascending q as one at a time would.  Away from the domain's
+    // edges no candidate is more than 3 cells away, less than half the
+    // domain when it spans 7 or more cells: there the periodic correction is
+    // the identity and that loop skips it.
+    auto scan = [&](uint32_t q0, uint32_t q1, auto &&d2_of) {
         q1 = min(q1, (uint32_t)a.n);
         uint32_t q = q0;
-        for (; q + 4 <= q1; q += 4) {
-            const float2 o0 = a.xy[q], o1 = a.xy[q + 1], o2 = a.xy[q + 2], o3 = a.xy[q + 3];
-            visit(q, o0);
-            visit(q + 1, o1);
-            visit(q + 2, o2);
-            visit(q + 3, o3);
+        const float2 *ptr = a.xy + q0;
+        for (; q + 4 <= q1; q += 4, ptr += 4) {
+            const float2 o0 = ptr[0], o1 = ptr[1], o2 = ptr[2], o3 = ptr[3];
+            const float e0 = d2_of(o0), e1 = d2_of(o1), e2 = d2_of(o2), e3 = d2_of(o3);
+            f(q, e0, e0 < a.r2 && (int)q != p);
+            f(q + 1, e1, e1 < a.r2 && (int)q + 1 != p);
+            f(q + 2, e2, e2 < a.r2 && (int)q + 2 != p);
+            f(q + 3, e3, e3 < a.r2 && (int)q + 3 != p);
         }
-        for (; q < q1; ++q) visit(q, a.xy[q]);
+        for (; q < q1; ++q, ++ptr) {
+            const float e = d2_of(*ptr);
+            f(q, e, e < a.r2 && (int)q != p);
+        }
     };
-    for (int oy = -1; oy <= 1; ++oy) {
-        const int ny = cy + oy < 0 ? cy + oy + a.gy : (cy + oy >= a.gy ? cy + oy - a.gy : cy + oy);
-        const uint32_t row = (uint32_t)(ny * a.gx);
-        if (cx == 0) {  // cells gx-1 | 0, 1
-            range(a.start[row + a.gx - 1], a.end[row + a.gx - 1]);
-            range(a.start[row], a.end[row + 1]);
-        } else if (cx == a.gx - 1) {  // cells gx-2, gx-1 | 0
-            range(a.start[row + cx - 1], a.end[row + cx]);
-            range(a.start[row], a.end[row]);
-        } else {
-            range(a.start[row + cx - 1], a.end[row + cx + 1]);
+    auto rows = [&](auto &&d2_of) {
+        for (int oy = -1; oy <= 1; ++oy) {
+            const int ny = cy + oy < 0 ? cy + oy + a.gy : (cy + oy >= a.gy ? cy + oy - a.gy : cy + oy);
+            const uint32_t row = (uint32_t)(ny * a.gx);
+            if (cx == 0) {  // cells gx-1 | 0, 1
+                scan(a.start[row + a.gx - 1], a.end[row + a.gx - 1], d2_of);
+                scan(a.start[row], a.end[row + 1], d2_of);
+            } else if (cx == a.gx - 1) {  // cells gx-2, gx-1 | 0
+                scan(a.start[row + cx - 1], a.end[row + cx], d2_of);
+                scan(a.start[row], a.end[row], d2_of);
+            } else {
+                scan(a.start[row + cx - 1], a.end[row + cx + 1], d2_of);
+            }
         }
+    };
+    const bool interior = cx >= 1 && cx <= a.gx - 2 && cy >= 1 && cy <= a.gy - 2 && a.gx >= 7 &&
+                          a.gy >= 7;
+    if (interior) {
+        rows([&](float2 o) {
+            const float dx = o.x - px, dy = o.y - py;
+            return dx * dx + dy * dy;
+        });
+    } else {
+        rows([&](float2 o) {
+            const float dx = periodic(o.x - px, a.W, a.half_w);
+            const float dy = periodic(o.y - py, a.H, a.half_h);
+            return dx * dx + dy * dy;
+        });
     }
 }
 
@@ -349,7 +360,7 @@ __global__ void k_nbr_count(NArgs a, uint32_t *__restrict__ counts) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.n) return;
     uint32_t c = 0;
-    for_neighbors(a, p, [&](uint32_t, float) { ++c; });
+    for_neighbors(a, p, [&](uint32_t, float, bool ok) { c += ok ? 1u : 0u; });
     const uint32_t i = a.id((uint32_t)p);
     if (i < (uint32_t)a.n) counts[i] = c;
 }
@@ -361,9 +372,11 @@ __global__ void k_nbr_fill(NArgs a, const uint32_t *__restrict__ offsets, uint32
     const uint32_t i = a.id((uint32_t)p);
     if (i >= (uint32_t)a.n) return;
     size_t o = offsets[i];
-    for_neighbors(a, p, [&](uint32_t q, float) {
-        if (o < capacity) nbr[o] = a.id(q);
-        ++o;
+    for_neighbors(a, p, [&](uint32_t q, float, bool ok) {
+        if (ok) {
+            if (o < capacity) nbr[o] = a.id(q);
+            ++o;
+        }
     });
 }
 
@@ -375,20 +388,22 @@ __global__ void k_density(NArgs a, float mass, float coef, float *__restrict__ r
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.n) return;
     float sum = 0.0f, comp = 0.0f;
-    auto add = [&](float d2) {
+    uint32_t cnt = 0;
+    // One Kahan step for an accepted term, computed for every candidate and
+    // kept by selects so that the loop does not branch.
+    auto add = [&](float d2, bool ok) {
         const float t = a.r2 - d2;
         const float w = mass * (coef * (t * t * t));
         const float yk = w - comp;
         const float s = sum + yk;
-        comp = (s - sum) - yk;
-        sum = s;
+        const float cn = (s - sum) - yk;
+        sum = ok ? s : sum;
+        comp = ok ? cn : comp;
+        cnt += ok ? 1u : 0u;
     };
-    add(0.0f);
-    uint32_t cnt = 0;
-    for_neighbors(a, p, [&](uint32_t, float d2) {
-        add(d2);
-        ++cnt;
-    });
+    add(0.0f, true);
+    cnt = 0;
+    for_neighbors(a, p, [&](uint32_t, float d2, bool ok) { add(d2, ok); });
     const uint32_t i = a.id((uint32_t)p);
     if (i < (uint32_t)a.n) {
         rho[i] = sum;
