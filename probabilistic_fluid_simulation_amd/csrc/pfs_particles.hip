@@ -135,10 +135,29 @@ int scan_exclusive(const uint32_t *in, uint32_t *out, int n, uint32_t *grand, ui
 
 // ---- stable LSD radix sort of (key, value) pairs ---------------------------
 // Digits of RB bits (8 or 10).  The histogram of block b is stored digit
-// major, hist[d * nb + b], so that its exclusive scan is every (digit,
-// block)'s first output position.
-template <int RB>
-__global__ __launch_bounds__(kThreads) void k_radix_hist(const uint32_t *__restrict__ keys, int n,
+// major, hist[d * nb + b]; each digit's row is then scanned on its own
+// (k_digit_rows), and the scatter adds the digit's base, the exclusive scan
+// of the row totals, which every scatter block forms for itself.
+
+// Cell of a position in [0, W): (int)(x * kx) with kx = gx / W (float),
+// clamped into the grid.
+__device__ __forceinline__ int cell_of(float x, float k, int g) {
+    const int c = (int)(x * k);
+    return c < 0 ? 0 : (c >= g ? g - 1 : c);
+}
+
+struct CellKeyArgs {  // the first pass computes the keys from the positions
+    const float *x, *y;
+    float kx, ky;
+    int gx, gy;
+};
+
+// Block histograms of one digit.  With FROM_XY the keys are made here from
+// the positions, cy * gx + cx, and written to `keys`.
+template <int RB, bool FROM_XY>
+__global__ __launch_bounds__(kThreads) void k_radix_hist(const uint32_t *__restrict__ kin,
+                                                         CellKeyArgs ck,
+                                                         uint32_t *__restrict__ keys, int n,
                                                          int shift, uint32_t *__restrict__ hist,
                                                          int nb) {
     constexpr int R = 1 << RB;
@@ -149,38 +168,92 @@ __global__ __launch_bounds__(kThreads) void k_radix_hist(const uint32_t *__restr
 #pragma unroll
     for (int r = 0; r < kItems; ++r) {
         const size_t i = base + (size_t)r * kThreads + threadIdx.x;
-        if (i < (size_t)n) atomicAdd(&h[(keys[i] >> shift) & (R - 1)], 1u);
+        if (i < (size_t)n) {
+            uint32_t k;
+            if (FROM_XY) {
+                k = (uint32_t)(cell_of(ck.y[i], ck.ky, ck.gy) * ck.gx + cell_of(ck.x[i], ck.kx, ck.gx));
+                keys[i] = k;
+            } else {
+                k = kin[i];
+            }
+            atomicAdd(&h[(k >> shift) & (R - 1)], 1u);
+        }
     }
     __syncthreads();
     for (int d = threadIdx.x; d < R; d += kThreads) hist[(size_t)d * nb + blockIdx.x] = h[d];
 }
 
-// Scatter of one pass.  Items are taken in tile order (round r covers
-// [r*kThreads, (r+1)*kThreads) of the block's tile), so the rank of an item
-// among equal digits is: the digit's offset for this block (the scan of the
-// digit-major histogram), plus the equal digits of earlier rounds and earlier
-// waves (LDS counts), plus the equal-digit lanes below it in its wave (RB
-// ballots, one per digit bit, intersected).
+// Block d scans row d of the digit-major histogram (nb block counts) in
+// place of `rows`, exclusively, and writes the row's total to tot[d].
+__global__ __launch_bounds__(kThreads) void k_digit_rows(const uint32_t *__restrict__ hist,
+                                                         uint32_t *__restrict__ rows,
+                                                         uint32_t *__restrict__ tot, int nb) {
+    __shared__ uint32_t sh[kWaves + 1];
+    const size_t row = (size_t)blockIdx.x * nb;
+    uint32_t carry = 0;
+    for (int b0 = 0; b0 < nb; b0 += kTile) {
+        const int j = b0 + threadIdx.x * kItems;
+        uint32_t v[kItems], s = 0;
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+            v[k] = j + k < nb ? hist[row + j + k] : 0u;
+            s += v[k];
+        }
+        uint32_t total;
+        uint32_t run = carry + block_excl_scan(s, sh, &total);
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+            if (j + k < nb) rows[row + j + k] = run;
+            run += v[k];
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0) tot[blockIdx.x] = carry;
+}
+
+// Scatter of one pass.  Wave w of block b takes the kItems * 64 keys
+// [b * kTile + w * 256, + 256) in kItems rounds of 64, so the tile's order is
+// (wave, round, lane).  The rank of an item among equal digits is: the
+// digit's base (scan of the row totals) plus its row scan at this block, plus
+// the equal digits of earlier waves (per-wave LDS counts, scanned across
+// waves once), plus those of this wave's earlier rounds (the wave's running
+// count) and of the lanes below it (RB ballots, one per digit bit,
+// intersected).  vin null: the values are the key indices.  xs non-null (the
+// last pass): the positions are gathered into sorted order as well.
 template <int RB>
 __global__ __launch_bounds__(kThreads) void k_radix_scatter(
     const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin, uint32_t *__restrict__ kout,
-    uint32_t *__restrict__ vout, int n, int shift, const uint32_t *__restrict__ offsets, int nb) {
+    uint32_t *__restrict__ vout, int n, int shift, const uint32_t *__restrict__ rows,
+    const uint32_t *__restrict__ tot, int nb, const float *__restrict__ x,
+    const float *__restrict__ y, float *__restrict__ xs, float *__restrict__ ys) {
     constexpr int R = 1 << RB;
+    static_assert(R <= kThreads, "one digit per thread");
     __shared__ uint32_t wcount[kWaves][R];
-    __shared__ uint32_t run[R], gofs[R];
+    __shared__ uint32_t gofs[R];
+    __shared__ uint32_t sh[kWaves + 1];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int d = threadIdx.x; d < R; d += kThreads) {
-        gofs[d] = offsets[(size_t)d * nb + blockIdx.x];
-        run[d] = 0;
+    for (int e = threadIdx.x; e < kWaves * R; e += kThreads) (&wcount[0][0])[e] = 0;
+    {
+        const uint32_t t = threadIdx.x < R ? tot[threadIdx.x] : 0u;
+        uint32_t all;
+        const uint32_t dbase = block_excl_scan(t, sh, &all);  // its barriers also order the clear
+        if (threadIdx.x < R) gofs[threadIdx.x] = dbase + rows[(size_t)threadIdx.x * nb + blockIdx.x];
     }
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const size_t base = (size_t)blockIdx.x * kTile;
+    const size_t base = (size_t)blockIdx.x * kTile + (size_t)wave * (kItems * 64) + lane;
+    uint32_t key[kItems], val[kItems], dig[kItems], local[kItems];
+#pragma unroll
     for (int r = 0; r < kItems; ++r) {
-        for (int e = threadIdx.x; e < kWaves * R; e += kThreads) (&wcount[0][0])[e] = 0;
-        const size_t i = base + (size_t)r * kThreads + threadIdx.x;
+        const size_t i = base + (size_t)r * 64;
         const bool valid = i < (size_t)n;
-        const uint32_t key = valid ? kin[i] : 0u, val = valid ? vin[i] : 0u;
-        const uint32_t d = (key >> shift) & (R - 1);
+        key[r] = valid ? kin[i] : 0u;
+        val[r] = valid ? (vin ? vin[i] : (uint32_t)i) : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < kItems; ++r) {
+        const bool valid = base + (size_t)r * 64 < (size_t)n;
+        const uint32_t d = (key[r] >> shift) & (R - 1);
+        dig[r] = d;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < RB; ++b) {
@@ -190,48 +263,41 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
         }
         if (!valid) peers = 0;
         const uint32_t rank = (uint32_t)__popcll(peers & below);
-        __syncthreads();  // wcount cleared; previous round's reads done
-        if (valid && rank == 0) wcount[wave][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        for (int dd = threadIdx.x; dd < R; dd += kThreads) {
-            uint32_t s = run[dd];
-            for (int w = 0; w < kWaves; ++w) {
-                const uint32_t c = wcount[w][dd];
-                wcount[w][dd] = s;
-                s += c;
-            }
-            run[dd] = s;
+        // The wave's own running count of digit d: read by every lane of
+        // the digit, advanced by its lowest lane.  A wave's LDS accesses
+        // complete in order, and the wave barrier keeps the compiler from
+        // moving the next round's read above this write.
+        const uint32_t before = valid ? wcount[wave][d] : 0u;
+        local[r] = before + rank;
+        if (valid && rank == 0) wcount[wave][d] = before + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    if (threadIdx.x < R) {  // digit counts of the waves -> exclusive across waves
+        uint32_t s = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t c = wcount[w][threadIdx.x];
+            wcount[w][threadIdx.x] = s;
+            s += c;
         }
-        __syncthreads();
-        if (valid) {
-            const uint32_t pos = gofs[d] + wcount[wave][d] + rank;
-            if (pos < (uint32_t)n) {  // always, when the scan is right: kept as a bound
-                kout[pos] = key;
-                vout[pos] = val;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kItems; ++r) {
+        if (base + (size_t)r * 64 >= (size_t)n) continue;
+        const uint32_t pos = gofs[dig[r]] + wcount[wave][dig[r]] + local[r];
+        if (pos < (uint32_t)n) {  // always, when the scans are right: kept as a bound
+            kout[pos] = key[r];
+            vout[pos] = val[r];
+            if (xs && val[r] < (uint32_t)n) {
+                xs[pos] = x[val[r]];
+                ys[pos] = y[val[r]];
             }
         }
-        __syncthreads();
     }
 }
 
 // ---- grid ------------------------------------------------------------------
-// Cell of a position in [0, W): (int)(x * kx) with kx = gx / W (float),
-// clamped into the grid.
-__device__ __forceinline__ int cell_of(float x, float k, int g) {
-    const int c = (int)(x * k);
-    return c < 0 ? 0 : (c >= g ? g - 1 : c);
-}
-
-// Cell keys, and the particle ids the sort carries as values.
-__global__ void k_cell_keys(const float *__restrict__ x, const float *__restrict__ y, int n,
-                            float kx, float ky, int gx, int gy, uint32_t *__restrict__ keys,
-                            uint32_t *__restrict__ ids) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    keys[i] = (uint32_t)(cell_of(y[i], ky, gy) * gx + cell_of(x[i], kx, gx));
-    ids[i] = (uint32_t)i;
-}
-
 // start[c] = the lower bound of cell c in the sorted keys (binary search;
 // the top levels of every search hit the same few cache lines), which is
 // also end[c - 1]; the last cell ends at n.
@@ -248,17 +314,6 @@ __global__ void k_cell_tables(const uint32_t *__restrict__ sk, int n, uint32_t *
     start[c] = lo;
     if (c > 0) end[c - 1] = lo;
     if (c + 1 == ncells) end[c] = (uint32_t)n;
-}
-
-__global__ void k_gather_xy(const uint32_t *__restrict__ order, const float *__restrict__ x,
-                            const float *__restrict__ y, float *__restrict__ xs,
-                            float *__restrict__ ys, int n) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const uint32_t i = order[p];
-    if (i >= (uint32_t)n) return;  // a permutation by construction: kept as a bound
-    xs[p] = x[i];
-    ys[p] = y[i];
 }
 
 __global__ void k_pack_xy(const float *__restrict__ xs, const float *__restrict__ ys,
@@ -460,13 +515,14 @@ RadixPlan radix_plan(uint32_t ncells) {
 }
 
 // Workspace layout (words): sorted keys (2 buffers), values spare buffer,
-// digit histogram and its scan (2^10 * nblocks each), scan block sums.
+// digit histogram and its row scans (2^10 * nblocks each), interleaved
+// positions, digit row totals (2^10), scan block sums.
 struct SortWs {
-    uint32_t *k0, *k1, *v1, *hist, *hscan, *sums;
+    uint32_t *k0, *k1, *v1, *hist, *hscan, *tot, *sums;
     float2 *xy;  // interleaved sorted positions for the neighbour passes
 };
 // Word offsets: k0 0, k1 n, v1 2n, hist 3n, hscan 3n + H, xy (float2,
-// rounded up to an even word) then sums.
+// rounded up to an even word), tot, then sums.
 size_t hist_words(int n) {
     const size_t nb = (size_t)nblocks_for((size_t)n);
     return ((size_t)1 << kMaxRadixBits) * (nb ? nb : 1);
@@ -476,7 +532,7 @@ size_t sort_ws_words(int n, size_t ncells) {
     const size_t hist = hist_words(n);
     size_t sums = std::max<size_t>(nblocks_for(hist), nblocks_for((size_t)n));
     sums = std::max<size_t>(sums, nblocks_for(ncells));
-    return xy_offset(n) + 2 * (size_t)n + sums + 64;
+    return xy_offset(n) + 2 * (size_t)n + ((size_t)1 << kMaxRadixBits) + sums + 64;
 }
 SortWs sort_ws(void *ws, int n) {  // ws: 8-byte aligned
     uint32_t *p = static_cast<uint32_t *>(ws);
@@ -487,7 +543,8 @@ SortWs sort_ws(void *ws, int n) {  // ws: 8-byte aligned
     s.hist = p + 3 * (size_t)n;
     s.hscan = s.hist + hist_words(n);
     s.xy = reinterpret_cast<float2 *>(p + xy_offset(n));
-    s.sums = p + xy_offset(n) + 2 * (size_t)n;
+    s.tot = p + xy_offset(n) + 2 * (size_t)n;
+    s.sums = s.tot + ((size_t)1 << kMaxRadixBits);
     return s;
 }
 
@@ -593,31 +650,45 @@ int pfs_ngrid_build(const pfs_ngrid *g, const float *x, const float *y, uint32_t
     const uint32_t ncells = (uint32_t)g->gx * (uint32_t)g->gy;
     SortWs w = sort_ws(ws, n);
 
-    // Values start as particle ids in the buffer that makes the last pass
-    // land in `order`.
     // Buffers alternate so that the last pass writes the keys to k0 (kept for
-    // the neighbour passes) and the particle ids to `order`.
+    // the neighbour passes) and the particle ids to `order`.  The first pass
+    // makes the keys (into `keys`) and takes the particle ids as its values;
+    // the last gathers the positions into sorted order.
     const RadixPlan plan = radix_plan(ncells);
     const int passes = plan.passes;
     uint32_t *kb[2] = {passes % 2 == 0 ? w.k0 : w.k1, passes % 2 == 0 ? w.k1 : w.k0};
     uint32_t *vb[2] = {passes % 2 == 0 ? order : w.v1, passes % 2 == 0 ? w.v1 : order};
-    hipLaunchKernelGGL(k_cell_keys, dim3(grid_for(n)), dim3(256), 0, s, x, y, n, g->kx, g->ky,
-                       g->gx, g->gy, keys, vb[0]);
+    const CellKeyArgs ck{x, y, g->kx, g->ky, g->gx, g->gy};
     const uint32_t *kin = keys;
     for (int p = 0; p < passes; ++p) {
         const int shift = plan.bits * p;
+        const bool last = p + 1 == passes;
+        const uint32_t *vin = p == 0 ? nullptr : vb[p & 1];
+        float *oxs = last ? xs : nullptr, *oys = last ? ys : nullptr;
         if (plan.bits == 8) {
-            hipLaunchKernelGGL(k_radix_hist<8>, dim3(nb), dim3(kThreads), 0, s, kin, n, shift, w.hist,
-                               nb);
-            PFS_TRY(scan_exclusive(w.hist, w.hscan, (1 << 8) * nb, nullptr, w.sums, s));
-            hipLaunchKernelGGL(k_radix_scatter<8>, dim3(nb), dim3(kThreads), 0, s, kin, vb[p & 1],
-                               kb[(p + 1) & 1], vb[(p + 1) & 1], n, shift, w.hscan, nb);
+            if (p == 0)
+                hipLaunchKernelGGL((k_radix_hist<8, true>), dim3(nb), dim3(kThreads), 0, s, nullptr,
+                                   ck, keys, n, shift, w.hist, nb);
+            else
+                hipLaunchKernelGGL((k_radix_hist<8, false>), dim3(nb), dim3(kThreads), 0, s, kin,
+                                   ck, nullptr, n, shift, w.hist, nb);
+            hipLaunchKernelGGL(k_digit_rows, dim3(1 << 8), dim3(kThreads), 0, s, w.hist, w.hscan,
+                               w.tot, nb);
+            hipLaunchKernelGGL(k_radix_scatter<8>, dim3(nb), dim3(kThreads), 0, s, kin, vin,
+                               kb[(p + 1) & 1], vb[(p + 1) & 1], n, shift, w.hscan, w.tot, nb, x, y,
+                               oxs, oys);
         } else {
-            hipLaunchKernelGGL(k_radix_hist<10>, dim3(nb), dim3(kThreads), 0, s, kin, n, shift,
-                               w.hist, nb);
-            PFS_TRY(scan_exclusive(w.hist, w.hscan, (1 << 10) * nb, nullptr, w.sums, s));
-            hipLaunchKernelGGL(k_radix_scatter<10>, dim3(nb), dim3(kThreads), 0, s, kin, vb[p & 1],
-                               kb[(p + 1) & 1], vb[(p + 1) & 1], n, shift, w.hscan, nb);
+            if (p == 0)
+                hipLaunchKernelGGL((k_radix_hist<10, true>), dim3(nb), dim3(kThreads), 0, s, nullptr,
+                                   ck, keys, n, shift, w.hist, nb);
+            else
+                hipLaunchKernelGGL((k_radix_hist<10, false>), dim3(nb), dim3(kThreads), 0, s, kin,
+                                   ck, nullptr, n, shift, w.hist, nb);
+            hipLaunchKernelGGL(k_digit_rows, dim3(1 << 10), dim3(kThreads), 0, s, w.hist, w.hscan,
+                               w.tot, nb);
+            hipLaunchKernelGGL(k_radix_scatter<10>, dim3(nb), dim3(kThreads), 0, s, kin, vin,
+                               kb[(p + 1) & 1], vb[(p + 1) & 1], n, shift, w.hscan, w.tot, nb, x, y,
+                               oxs, oys);
         }
         kin = kb[(p + 1) & 1];
     }
@@ -626,7 +697,6 @@ int pfs_ngrid_build(const pfs_ngrid *g, const float *x, const float *y, uint32_t
     const unsigned cblocks = (unsigned)(((size_t)ncells + 255) / 256);
     hipLaunchKernelGGL(k_cell_tables, dim3(cblocks), dim3(256), 0, s, w.k0, n, cell_start, cell_end,
                        ncells);
-    hipLaunchKernelGGL(k_gather_xy, dim3(grid_for(n)), dim3(256), 0, s, order, x, y, xs, ys, n);
     return check_launch("neighbour grid");
 }
 
