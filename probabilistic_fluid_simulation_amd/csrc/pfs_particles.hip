@@ -211,6 +211,15 @@ __global__ __launch_bounds__(kThreads) void k_digit_rows(const uint32_t *__restr
     if (threadIdx.x == 0) tot[blockIdx.x] = carry;
 }
 
+// Workgroups are dispatched to the 8 XCDs in turn (workgroup i to XCD i % 8);
+// this numbers tiles so that each XCD takes a contiguous run of them.  Tiles
+// next to each other write the two ends of the same output cache lines, which
+// then meet in one L2.  A bijection on [0, nb) for any nb.
+__device__ __forceinline__ int xcd_block(int i, int nb) {
+    const int q = nb >> 3, r = nb & 7, x = i & 7;
+    return x * q + min(x, r) + (i >> 3);
+}
+
 // Scatter of one pass.  Wave w of block b takes the kItems * 64 keys
 // [b * kTile + w * 256, + 256) in kItems rounds of 64, so the tile's order is
 // (wave, round, lane).  The rank of an item among equal digits is: the
@@ -232,15 +241,16 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
     __shared__ uint32_t gofs[R];
     __shared__ uint32_t sh[kWaves + 1];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tb = xcd_block(blockIdx.x, nb);  // the tile of this workgroup
     for (int e = threadIdx.x; e < kWaves * R; e += kThreads) (&wcount[0][0])[e] = 0;
     {
         const uint32_t t = threadIdx.x < R ? tot[threadIdx.x] : 0u;
         uint32_t all;
         const uint32_t dbase = block_excl_scan(t, sh, &all);  // its barriers also order the clear
-        if (threadIdx.x < R) gofs[threadIdx.x] = dbase + rows[(size_t)threadIdx.x * nb + blockIdx.x];
+        if (threadIdx.x < R) gofs[threadIdx.x] = dbase + rows[(size_t)threadIdx.x * nb + tb];
     }
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const size_t base = (size_t)blockIdx.x * kTile + (size_t)wave * (kItems * 64) + lane;
+    const size_t base = (size_t)tb * kTile + (size_t)wave * (kItems * 64) + lane;
     uint32_t key[kItems], val[kItems], dig[kItems], local[kItems];
 #pragma unroll
     for (int r = 0; r < kItems; ++r) {
