@@ -238,7 +238,8 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
     constexpr int R = 1 << RB;
     static_assert(R <= kThreads, "one digit per thread");
     __shared__ uint32_t wcount[kWaves][R];
-    __shared__ uint32_t gofs[R];
+    __shared__ uint32_t gofs[R], loff[R];
+    __shared__ uint32_t lkey[kTile], lval[kTile];
     __shared__ uint32_t sh[kWaves + 1];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int tb = xcd_block(blockIdx.x, nb);  // the tile of this workgroup
@@ -283,25 +284,44 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
+    uint32_t dtotal = 0;
     if (threadIdx.x < R) {  // digit counts of the waves -> exclusive across waves
-        uint32_t s = 0;
         for (int w = 0; w < kWaves; ++w) {
             const uint32_t c = wcount[w][threadIdx.x];
-            wcount[w][threadIdx.x] = s;
-            s += c;
+            wcount[w][threadIdx.x] = dtotal;
+            dtotal += c;
         }
     }
-    __syncthreads();
+    // The tile is first put in digit order in LDS (loff[d]: the tile's first
+    // slot of digit d), then written out by consecutive threads, so that each
+    // store instruction covers a few runs of consecutive positions.
+    {
+        uint32_t all;
+        const uint32_t lo = block_excl_scan(dtotal, sh, &all);
+        if (threadIdx.x < R) loff[threadIdx.x] = lo;
+        __syncthreads();
+    }
 #pragma unroll
     for (int r = 0; r < kItems; ++r) {
         if (base + (size_t)r * 64 >= (size_t)n) continue;
-        const uint32_t pos = gofs[dig[r]] + wcount[wave][dig[r]] + local[r];
+        const uint32_t li = loff[dig[r]] + wcount[wave][dig[r]] + local[r];
+        if (li < (uint32_t)kTile) {  // always: kept as a bound
+            lkey[li] = key[r];
+            lval[li] = val[r];
+        }
+    }
+    __syncthreads();
+    const int count = (int)min((size_t)kTile, (size_t)n - (size_t)tb * kTile);
+    for (int j = threadIdx.x; j < count; j += kThreads) {
+        const uint32_t k = lkey[j], v = lval[j];
+        const uint32_t d = (k >> shift) & (R - 1);
+        const uint32_t pos = gofs[d] + ((uint32_t)j - loff[d]);
         if (pos < (uint32_t)n) {  // always, when the scans are right: kept as a bound
-            kout[pos] = key[r];
-            vout[pos] = val[r];
-            if (xs && val[r] < (uint32_t)n) {
-                xs[pos] = x[val[r]];
-                ys[pos] = y[val[r]];
+            kout[pos] = k;
+            vout[pos] = v;
+            if (xs && v < (uint32_t)n) {
+                xs[pos] = x[v];
+                ys[pos] = y[v];
             }
         }
     }
