@@ -257,9 +257,10 @@ int pfs_ngrid_neighbors(const pfs_ngrid *g, const float *xs, const float *ys, co
                         const uint32_t *cell_start, const uint32_t *cell_end, uint32_t *counts,
                         uint32_t *offsets, uint32_t *nbr, size_t nbr_capacity, uint32_t *total,
                         void *workspace, size_t workspace_bytes, void *stream);
-/* rho[i] = sum over i and its neighbours of mass * 4/(pi r^8) * (r^2 - d^2)^3
- * (2D poly6), Kahan-compensated FP32 in the neighbour order, self first;
- * counts (may be NULL) gets the neighbour counts of the same traversal. */
+/* rho[i] = (mass * 4/(pi r^8)) * S, S the FP32 sum over i and its neighbours
+ * of (r^2 - d^2)^3 (2D poly6) in the traversal order (i where its cell's
+ * range meets it); counts (may be NULL) gets the neighbour counts of the same
+ * traversal. */
 int pfs_ngrid_density(const pfs_ngrid *g, const float *xs, const float *ys, const uint32_t *order,
                       const uint32_t *cell_start, const uint32_t *cell_end, float mass, float *rho,
                       uint32_t *counts, void *workspace, size_t workspace_bytes, void *stream);

@@ -118,25 +118,18 @@ long long orcp_ngrid_neighbors(const float *xs, const float *ys, const uint32_t 
     return total;
 }
 
-/* SPH density (2D poly6), Kahan-compensated in traversal order, self first. */
+/* SPH density (2D poly6): rho = (mass * coef) * S, S the float sum in
+ * traversal order of (r^2 - d2)^3 over the candidates with d2 < r^2, the
+ * particle itself included where the traversal meets it. */
 void orcp_ngrid_density(const float *xs, const float *ys, const uint32_t *order, int n,
                         const uint32_t *keys_sorted, const uint32_t *start, const uint32_t *end,
                         int gx, int gy, float W, float H, float r, float mass, float *rho) {
     const float half_w = W * 0.5f, half_h = H * 0.5f, r2 = r * r;
     const double rd = r;
     const float coef = (float)(4.0 / (3.14159265358979323846 * rd * rd * rd * rd * rd * rd * rd * rd));
+    const float mc = mass * coef;
     for (int p = 0; p < n; ++p) {
-        float sum = 0.0f, comp = 0.0f;
-#define ADD(d2)                                      \
-    do {                                             \
-        const float t = r2 - (d2);                   \
-        const float wv = mass * (coef * (t * t * t)); \
-        const float yk = wv - comp;                  \
-        const float s = sum + yk;                    \
-        comp = (s - sum) - yk;                       \
-        sum = s;                                     \
-    } while (0)
-        ADD(0.0f);
+        float sum = 0.0f;
         const uint32_t c = keys_sorted[p];
         const int cx = (int)(c % (uint32_t)gx), cy = (int)(c / (uint32_t)gx);
         const float px = xs[p], py = ys[p];
@@ -146,15 +139,16 @@ void orcp_ngrid_density(const float *xs, const float *ys, const uint32_t *order,
                 const int nx = (cx + ox + gx) % gx;
                 const uint32_t cc = (uint32_t)(ny * gx + nx);
                 for (uint32_t q = start[cc]; q < end[cc]; ++q) {
-                    if ((int)q == p) continue;
                     const float dx = periodic(xs[q] - px, W, half_w);
                     const float dy = periodic(ys[q] - py, H, half_h);
                     const float d2 = dx * dx + dy * dy;
-                    if (d2 < r2) ADD(d2);
+                    if (d2 < r2) {
+                        const float t = r2 - d2;
+                        sum += t * t * t;
+                    }
                 }
             }
         }
-#undef ADD
-        rho[order ? order[p] : (uint32_t)p] = sum;
+        rho[order ? order[p] : (uint32_t)p] = mc * sum;
     }
 }

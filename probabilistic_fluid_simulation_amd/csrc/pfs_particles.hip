@@ -18,7 +18,7 @@
 //                            lists (CSR) of every particle within radius r,
 //                            visiting the 3x3 cells in a fixed order;
 //   pfs_ngrid_density        SPH density from the lists' traversal with the
-//                            poly6 kernel, Kahan-compensated FP32 sums.
+//                            poly6 kernel, FP32 sums in traversal order.
 //
 // Every integer output (keys, sort order, tables, lists) is deterministic
 // and equals the CPU restatement in oracle/pfs_particles.c bit for bit.
@@ -375,9 +375,10 @@ struct NArgs {
     __device__ uint32_t id(uint32_t p) const { return order ? order[p] : p; }
 };
 
-// Visits every particle q of the 3x3 cells around p's cell (rows then
-// columns, -1..1, then q ascending): f(q, d2, ok) with ok = q != p and
-// d2 < r^2, so that callers can accumulate without branching.  The cells of
+// Visits every particle q of the 3x3 cells around p's cell, p included (rows
+// then columns, -1..1, then q ascending): f(q, d2) with d2 the squared
+// periodic distance, for every candidate, so that callers accumulate without
+// branching.  The cells of
 // one row are consecutive keys, so the three are one range of sorted
 // positions, [start of the left cell, end of the right cell), split in two
 // where the row wraps.
@@ -400,15 +401,12 @@ __device__ __forceinline__ void for_neighbors(const NArgs &a, int p, F &&f) {
         for (; q + 4 <= q1; q += 4, ptr += 4) {
             const float2 o0 = ptr[0], o1 = ptr[1], o2 = ptr[2], o3 = ptr[3];
             const float e0 = d2_of(o0), e1 = d2_of(o1), e2 = d2_of(o2), e3 = d2_of(o3);
-            f(q, e0, e0 < a.r2 && (int)q != p);
-            f(q + 1, e1, e1 < a.r2 && (int)q + 1 != p);
-            f(q + 2, e2, e2 < a.r2 && (int)q + 2 != p);
-            f(q + 3, e3, e3 < a.r2 && (int)q + 3 != p);
+            f(q, e0);
+            f(q + 1, e1);
+            f(q + 2, e2);
+            f(q + 3, e3);
         }
-        for (; q < q1; ++q, ++ptr) {
-            const float e = d2_of(*ptr);
-            f(q, e, e < a.r2 && (int)q != p);
-        }
+        for (; q < q1; ++q, ++ptr) f(q, d2_of(*ptr));
     };
     auto rows = [&](auto &&d2_of) {
         for (int oy = -1; oy <= 1; ++oy) {
@@ -445,7 +443,7 @@ __global__ void k_nbr_count(NArgs a, uint32_t *__restrict__ counts) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.n) return;
     uint32_t c = 0;
-    for_neighbors(a, p, [&](uint32_t, float, bool ok) { c += ok ? 1u : 0u; });
+    for_neighbors(a, p, [&](uint32_t q, float d2) { c += d2 < a.r2 && (int)q != p ? 1u : 0u; });
     const uint32_t i = a.id((uint32_t)p);
     if (i < (uint32_t)a.n) counts[i] = c;
 }
@@ -457,42 +455,36 @@ __global__ void k_nbr_fill(NArgs a, const uint32_t *__restrict__ offsets, uint32
     const uint32_t i = a.id((uint32_t)p);
     if (i >= (uint32_t)a.n) return;
     size_t o = offsets[i];
-    for_neighbors(a, p, [&](uint32_t q, float, bool ok) {
-        if (ok) {
+    for_neighbors(a, p, [&](uint32_t q, float d2) {
+        if (d2 < a.r2 && (int)q != p) {
             if (o < capacity) nbr[o] = a.id(q);
             ++o;
         }
     });
 }
 
-// SPH density with the 2D poly6 kernel, W(d) = c (r^2 - d^2)^3, summed with
-// Kahan compensation in the traversal order, self term first; with counts,
+// SPH density with the 2D poly6 kernel, W(d) = c (r^2 - d^2)^3; with counts,
 // the neighbour counts of the same traversal.
-__global__ void k_density(NArgs a, float mass, float coef, float *__restrict__ rho,
+// rho = mc * S with mc = mass * 4/(pi r^8) and S the FP32 sum, in traversal
+// order and p included where it falls, of (r^2 - d2)^3 over the candidates
+// with d2 < r^2.  A rejected candidate adds +0 (t clamped at 0), which leaves
+// S unchanged, so the loop does not branch.  All terms are positive: S is
+// within (m - 1) u of the exact sum of its m rounded terms.
+__global__ void k_density(NArgs a, float mc, float *__restrict__ rho,
                           uint32_t *__restrict__ counts) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.n) return;
-    float sum = 0.0f, comp = 0.0f;
+    float sum = 0.0f;
     uint32_t cnt = 0;
-    // One Kahan step for an accepted term, computed for every candidate and
-    // kept by selects so that the loop does not branch.
-    auto add = [&](float d2, bool ok) {
-        const float t = a.r2 - d2;
-        const float w = mass * (coef * (t * t * t));
-        const float yk = w - comp;
-        const float s = sum + yk;
-        const float cn = (s - sum) - yk;
-        sum = ok ? s : sum;
-        comp = ok ? cn : comp;
-        cnt += ok ? 1u : 0u;
-    };
-    add(0.0f, true);
-    cnt = 0;
-    for_neighbors(a, p, [&](uint32_t, float d2, bool ok) { add(d2, ok); });
+    for_neighbors(a, p, [&](uint32_t, float d2) {
+        const float t = fmaxf(a.r2 - d2, 0.0f);
+        sum += t * t * t;
+        cnt += d2 < a.r2 ? 1u : 0u;
+    });
     const uint32_t i = a.id((uint32_t)p);
     if (i < (uint32_t)a.n) {
-        rho[i] = sum;
-        if (counts) counts[i] = cnt;
+        rho[i] = mc * sum;
+        if (counts) counts[i] = cnt - 1u;  // p itself (d2 = 0) was counted
     }
 }
 
@@ -772,10 +764,11 @@ int pfs_ngrid_density(const pfs_ngrid *g, const float *xs, const float *ys, cons
     hipStream_t s = (hipStream_t)stream;
     const NArgs a = nargs(g, xs, ys, w.xy, order, w.k0, cell_start, cell_end);
     hipLaunchKernelGGL(k_pack_xy, dim3(grid_for(g->n)), dim3(256), 0, s, xs, ys, w.xy, g->n);
-    // 2D poly6: 4 / (pi r^8), in float from the double constant.
+    // 2D poly6: 4 / (pi r^8), in float from the double constant, times the mass.
     const double r = g->radius;
     const float coef = (float)(4.0 / (3.14159265358979323846 * r * r * r * r * r * r * r * r));
-    hipLaunchKernelGGL(k_density, dim3(grid_for(g->n)), dim3(256), 0, s, a, mass, coef, rho, counts);
+    const float mc = mass * coef;
+    hipLaunchKernelGGL(k_density, dim3(grid_for(g->n)), dim3(256), 0, s, a, mc, rho, counts);
     return check_launch("density");
 }
 

@@ -1,6 +1,6 @@
 """The particle layer on the GPU (pfs_particles_advect, pfs_ngrid_*,
 pfs_scan_exclusive_u32) against its CPU oracle: integer outputs and
-positions bit-exact, densities bit-exact (same Kahan order)."""
+positions bit-exact, densities bit-exact (same summation order)."""
 from __future__ import annotations
 
 import numpy as np

@@ -113,16 +113,24 @@ def test_neighbor_order_is_cell_rows_then_columns_then_sorted_position():
 
 
 def test_density_against_fp64_brute_force():
+    """FP64 poly6 sums of the same float32 squared distances.  Every term is
+    positive, so the float32 result (rounded r^2 - d2, two products, the sum
+    of m terms, the final scaling) is within (m + 6) u of it, u = 2^-24."""
     W = H = 32.0
-    r = 2.5
-    x, y = _random(700, W, H, 8)
-    g, og = _oracle(x, y, W, H, r)
-    rho = og.density(1.0)
-    sets, d2 = _brute_neighbors(x, y, W, H, r)
-    coef = 4.0 / (np.pi * r ** 8)
-    want = np.array([coef * (r * r) ** 3 + sum(coef * (r * r - float(d2[i, j])) ** 3
-                                               for j in sets[i]) for i in range(len(x))])
-    np.testing.assert_allclose(rho.astype(np.float64), want, rtol=2e-6)
+    for r in (2.5, 1.7):  # r^2 exact in float32, then not
+        x, y = _random(700, W, H, 8)
+        g, og = _oracle(x, y, W, H, r)
+        rho = og.density(1.0).astype(np.float64)
+        sets, d2 = _brute_neighbors(x, y, W, H, r)
+        r2 = float(np.float32(r) * np.float32(r))
+        coef = 4.0 / (np.pi * float(np.float32(r)) ** 8)
+        want = np.array([coef * (r2 ** 3 + sum((r2 - float(d2[i, j])) ** 3 for j in sets[i]))
+                         for i in range(len(x))])
+        m = np.array([len(sv) + 1 for sv in sets])
+        bound = (m + 6) * 2.0 ** -24 * want
+        assert (np.abs(rho - want) <= bound).all(), np.max(np.abs(rho - want) / want)
+        # ... and a dropped self term or neighbour would be far outside it
+        assert (np.abs(rho - coef * r2 ** 3 - want) > bound).all()
 
 
 def test_tracers_integer_shift_is_exact():
