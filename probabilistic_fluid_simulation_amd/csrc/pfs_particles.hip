@@ -346,12 +346,6 @@ __global__ void k_cell_tables(const uint32_t *__restrict__ sk, int n, uint32_t *
     if (c + 1 == ncells) end[c] = (uint32_t)n;
 }
 
-__global__ void k_pack_xy(const float *__restrict__ xs, const float *__restrict__ ys,
-                          float2 *__restrict__ xy, int n) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < n) xy[p] = make_float2(xs[p], ys[p]);
-}
-
 __global__ void k_gather_u32(const uint32_t *__restrict__ index, const uint32_t *__restrict__ in,
                              uint32_t *__restrict__ out, int n) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -367,7 +361,6 @@ __device__ __forceinline__ float periodic(float d, float L, float half) {
 
 struct NArgs {
     const float *xs, *ys;
-    const float2 *xy;       // (xs, ys) interleaved, read by the candidate loops
     const uint32_t *order;  // null: outputs indexed by sorted position
     const uint32_t *sk, *start, *end;
     int n, gx, gy;
@@ -387,8 +380,7 @@ __device__ __forceinline__ void for_neighbors(const NArgs &a, int p, F &&f) {
     const uint32_t c = a.sk[p];
     if (c >= (uint32_t)a.gx * (uint32_t)a.gy) return;  // a cell by construction: kept as a bound
     const int cx = (int)(c % (uint32_t)a.gx), cy = (int)(c / (uint32_t)a.gx);
-    const float2 pp = a.xy[p];
-    const float px = pp.x, py = pp.y;
+    const float px = a.xs[p], py = a.ys[p];
     // Candidates are taken four at a time, their loads issued together, and
     // visited in ascending q as one at a time would.  Away from the domain's
     // edges no candidate is more than 3 cells away, less than half the
@@ -397,16 +389,18 @@ __device__ __forceinline__ void for_neighbors(const NArgs &a, int p, F &&f) {
     auto scan = [&](uint32_t q0, uint32_t q1, auto &&d2_of) {
         q1 = min(q1, (uint32_t)a.n);
         uint32_t q = q0;
-        const float2 *ptr = a.xy + q0;
-        for (; q + 4 <= q1; q += 4, ptr += 4) {
-            const float2 o0 = ptr[0], o1 = ptr[1], o2 = ptr[2], o3 = ptr[3];
-            const float e0 = d2_of(o0), e1 = d2_of(o1), e2 = d2_of(o2), e3 = d2_of(o3);
+        const float *xp = a.xs + q0, *yp = a.ys + q0;
+        for (; q + 4 <= q1; q += 4, xp += 4, yp += 4) {
+            const float x0 = xp[0], x1 = xp[1], x2 = xp[2], x3 = xp[3];
+            const float y0 = yp[0], y1 = yp[1], y2 = yp[2], y3 = yp[3];
+            const float e0 = d2_of(x0, y0), e1 = d2_of(x1, y1), e2 = d2_of(x2, y2),
+                        e3 = d2_of(x3, y3);
             f(q, e0);
             f(q + 1, e1);
             f(q + 2, e2);
             f(q + 3, e3);
         }
-        for (; q < q1; ++q, ++ptr) f(q, d2_of(*ptr));
+        for (; q < q1; ++q, ++xp, ++yp) f(q, d2_of(*xp, *yp));
     };
     auto rows = [&](auto &&d2_of) {
         for (int oy = -1; oy <= 1; ++oy) {
@@ -426,14 +420,14 @@ __device__ __forceinline__ void for_neighbors(const NArgs &a, int p, F &&f) {
     const bool interior = cx >= 1 && cx <= a.gx - 2 && cy >= 1 && cy <= a.gy - 2 && a.gx >= 7 &&
                           a.gy >= 7;
     if (interior) {
-        rows([&](float2 o) {
-            const float dx = o.x - px, dy = o.y - py;
+        rows([&](float ox, float oy) {
+            const float dx = ox - px, dy = oy - py;
             return dx * dx + dy * dy;
         });
     } else {
-        rows([&](float2 o) {
-            const float dx = periodic(o.x - px, a.W, a.half_w);
-            const float dy = periodic(o.y - py, a.H, a.half_h);
+        rows([&](float ox, float oy) {
+            const float dx = periodic(ox - px, a.W, a.half_w);
+            const float dy = periodic(oy - py, a.H, a.half_h);
             return dx * dx + dy * dy;
         });
     }
@@ -537,24 +531,20 @@ RadixPlan radix_plan(uint32_t ncells) {
 }
 
 // Workspace layout (words): sorted keys (2 buffers), values spare buffer,
-// digit histogram and its row scans (2^10 * nblocks each), interleaved
-// positions, digit row totals (2^10), scan block sums.
+// digit histogram and its row scans (2^10 * nblocks each), digit row totals
+// (2^10), scan block sums.
 struct SortWs {
     uint32_t *k0, *k1, *v1, *hist, *hscan, *tot, *sums;
-    float2 *xy;  // interleaved sorted positions for the neighbour passes
 };
-// Word offsets: k0 0, k1 n, v1 2n, hist 3n, hscan 3n + H, xy (float2,
-// rounded up to an even word), tot, then sums.
 size_t hist_words(int n) {
     const size_t nb = (size_t)nblocks_for((size_t)n);
     return ((size_t)1 << kMaxRadixBits) * (nb ? nb : 1);
 }
-size_t xy_offset(int n) { return (3 * (size_t)n + 2 * hist_words(n) + 1) & ~(size_t)1; }
 size_t sort_ws_words(int n, size_t ncells) {
     const size_t hist = hist_words(n);
     size_t sums = std::max<size_t>(nblocks_for(hist), nblocks_for((size_t)n));
     sums = std::max<size_t>(sums, nblocks_for(ncells));
-    return xy_offset(n) + 2 * (size_t)n + ((size_t)1 << kMaxRadixBits) + sums + 64;
+    return 3 * (size_t)n + 2 * hist + ((size_t)1 << kMaxRadixBits) + sums + 64;
 }
 SortWs sort_ws(void *ws, int n) {  // ws: 8-byte aligned
     uint32_t *p = static_cast<uint32_t *>(ws);
@@ -564,8 +554,7 @@ SortWs sort_ws(void *ws, int n) {  // ws: 8-byte aligned
     s.v1 = p + 2 * (size_t)n;
     s.hist = p + 3 * (size_t)n;
     s.hscan = s.hist + hist_words(n);
-    s.xy = reinterpret_cast<float2 *>(p + xy_offset(n));
-    s.tot = p + xy_offset(n) + 2 * (size_t)n;
+    s.tot = s.hscan + hist_words(n);
     s.sums = s.tot + ((size_t)1 << kMaxRadixBits);
     return s;
 }
@@ -576,12 +565,11 @@ bool valid_grid(const pfs_ngrid *g) {
            g->radius <= g->domain_w / g->gx && g->radius <= g->domain_h / g->gy;
 }
 
-NArgs nargs(const pfs_ngrid *g, const float *xs, const float *ys, const float2 *xy,
+NArgs nargs(const pfs_ngrid *g, const float *xs, const float *ys,
             const uint32_t *order, const uint32_t *sk, const uint32_t *start, const uint32_t *end) {
     NArgs a{};
     a.xs = xs;
     a.ys = ys;
-    a.xy = xy;
     a.order = order;
     a.sk = sk;
     a.start = start;
@@ -734,8 +722,7 @@ int pfs_ngrid_neighbors(const pfs_ngrid *g, const float *xs, const float *ys, co
                          "ngrid: workspace not 8-byte aligned or smaller than pfs_ngrid_workspace_bytes()");
     hipStream_t s = (hipStream_t)stream;
     SortWs w = sort_ws(ws, g->n);
-    const NArgs a = nargs(g, xs, ys, w.xy, order, w.k0, cell_start, cell_end);
-    hipLaunchKernelGGL(k_pack_xy, dim3(grid_for(g->n)), dim3(256), 0, s, xs, ys, w.xy, g->n);
+    const NArgs a = nargs(g, xs, ys, order, w.k0, cell_start, cell_end);
     hipLaunchKernelGGL(k_nbr_count, dim3(grid_for(g->n)), dim3(256), 0, s, a, counts);
     PFS_TRY(scan_exclusive(counts, offsets, g->n, total, w.sums, s));
     if (!nbr) return PFS_OK;  // counts and offsets only
@@ -762,8 +749,7 @@ int pfs_ngrid_density(const pfs_ngrid *g, const float *xs, const float *ys, cons
                          "ngrid: workspace not 8-byte aligned or smaller than pfs_ngrid_workspace_bytes()");
     SortWs w = sort_ws(ws, g->n);
     hipStream_t s = (hipStream_t)stream;
-    const NArgs a = nargs(g, xs, ys, w.xy, order, w.k0, cell_start, cell_end);
-    hipLaunchKernelGGL(k_pack_xy, dim3(grid_for(g->n)), dim3(256), 0, s, xs, ys, w.xy, g->n);
+    const NArgs a = nargs(g, xs, ys, order, w.k0, cell_start, cell_end);
     // 2D poly6: 4 / (pi r^8), in float from the double constant, times the mass.
     const double r = g->radius;
     const float coef = (float)(4.0 / (3.14159265358979323846 * r * r * r * r * r * r * r * r));
