@@ -39,17 +39,23 @@ def algorithmic_bytes(n_cells: int, n_pixels: int, k: int) -> dict:
             "pressure_dye": (20 + 12 * k) * n_cells + 32 * n_pixels + 8 * n_cells}
 
 
-def pmc_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary (calibrated against a copy of known size),
-    or None if there is none."""
+def pmc_figure(name: str, kernel: str):
+    """A per-launch figure of `kernel` from the committed rocprofv3 counter
+    summary: "traffic_bytes_per_launch" (HBM bytes, FETCH_SIZE/WRITE_SIZE
+    calibrated against a copy of known size) or "valu_insts_per_launch"
+    (SQ_INSTS_VALU); None if there is none."""
     path = os.path.join(ROOT, "profiles", "plain-1024x1024-K8_pmc.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith(f"traffic_bytes_per_launch k_{kernel} = "):
+        if line.startswith(f"{name} k_{kernel} = "):
             return int(line.rsplit("=", 1)[1])
     return None
+
+
+# VALU issue rate: a wave64 FP32 instruction holds a 16-lane SIMD for 4 cycles;
+# 256 CUs x 4 SIMDs at 2.4 GHz.
+VALU_WAVE_INSTS_PER_S = 256 * 4 * 2.4e9 / 4
 
 
 def cpu_baseline(scene, budget_steps: int) -> dict:
@@ -452,11 +458,17 @@ def main() -> int:
                    "two_call_ms_per_step": wall_two / args.steps * 1e3},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": pmc_traffic(dom),
+                     "traffic": pmc_figure("traffic_bytes_per_launch", dom),
                      "traffic_source": "profiles/plain-1024x1024-K8_pmc.md (rocprofv3 FETCH_SIZE x2 + "
                                        "WRITE_SIZE x1, calibrated on a 256 MiB copy)",
                      "algorithmic_bytes_per_launch": alg[dom]},
     }
+    valu = pmc_figure("valu_insts_per_launch", dom)
+    if valu is not None:
+        # The kernel's other bound: the time its VALU instructions take to
+        # issue, as a fraction of its measured duration.
+        out["roofline"]["valu_issue_frac"] = valu / VALU_WAVE_INSTS_PER_S / (avg_ms[dom] * 1e-3)
+        out["roofline"]["valu_insts_per_launch"] = valu
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scenes.plain_1m(steps=args.steps), args.cpu_steps)
     del vp, tmp, im, itmp
