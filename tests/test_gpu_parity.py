@@ -196,6 +196,22 @@ def test_plain_1m_bitexact(orc, combined):
     assert_bitexact(gpu_run(s, 3, combined=combined), oracle_run(orc, s, 3))
 
 
+@pytest.mark.parametrize("iters", [16, 30])
+def test_plain_1m_many_sweeps_bitexact(orc, iters):
+    """The 1024x1024 workload at K = 16 and at the reference's default K = 30
+    (the sweeps run in chunks of 8 through the iterate planes): 2 steps
+    bit-exact."""
+    s = scenes.plain_1m()
+    assert_bitexact(gpu_run(s, 2, iters=iters, combined=True), oracle_run(orc, s, 2, iters=iters))
+
+
+def test_plain_step_at_2048_bitexact(orc):
+    """The plain step on the 2048x2048 field and dye (1024 tiles), K = 16:
+    one step bit-exact."""
+    s = scenes.precision_4m()
+    assert_bitexact(gpu_run(s, 1, combined=True), oracle_run(orc, s, 1))
+
+
 def test_stream_and_repeat_determinism(orc):
     s = scenes.random_scene(150, 90, 150, 90, seed=13, dt=0.7, visc=0.2, iters=8)
     st = torch.cuda.Stream()
