@@ -237,7 +237,7 @@ typedef struct pfs_ngrid {
 int pfs_particles_advect(float *x, float *y, int n, const pfs_field *vp, float dt, void *stream);
 /* Fills *g: gx = floor(W / r) (lowered until cells are >= r wide), likewise gy. */
 int pfs_ngrid_init(pfs_ngrid *g, int n, float domain_w, float domain_h, float radius);
-/* Workspaces of pfs_ngrid_* are 8-byte aligned, of this many bytes. */
+/* Workspaces of pfs_ngrid_* are 4-byte aligned, of this many bytes. */
 size_t pfs_ngrid_workspace_bytes(const pfs_ngrid *g);
 /* keys[n]: cell of particle i; order[n]: particle ids sorted by (key, id);
  * cell_start/cell_end[gx*gy]: [start, end) of each cell in sorted order

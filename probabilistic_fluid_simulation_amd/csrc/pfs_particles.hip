@@ -183,8 +183,9 @@ __global__ __launch_bounds__(kThreads) void k_radix_hist(const uint32_t *__restr
     for (int d = threadIdx.x; d < R; d += kThreads) hist[(size_t)d * nb + blockIdx.x] = h[d];
 }
 
-// Block d scans row d of the digit-major histogram (nb block counts) in
-// place of `rows`, exclusively, and writes the row's total to tot[d].
+// Block d scans row d of the digit-major histogram (nb block counts)
+// exclusively into the same row of `rows`, and writes the row's total to
+// tot[d].
 __global__ __launch_bounds__(kThreads) void k_digit_rows(const uint32_t *__restrict__ hist,
                                                          uint32_t *__restrict__ rows,
                                                          uint32_t *__restrict__ tot, int nb) {
@@ -371,10 +372,9 @@ struct NArgs {
 // Visits every particle q of the 3x3 cells around p's cell, p included (rows
 // then columns, -1..1, then q ascending): f(q, d2) with d2 the squared
 // periodic distance, for every candidate, so that callers accumulate without
-// branching.  The cells of
-// one row are consecutive keys, so the three are one range of sorted
-// positions, [start of the left cell, end of the right cell), split in two
-// where the row wraps.
+// branching.  The cells of one row are consecutive keys, so the three are
+// one range of sorted positions, [start of the left cell, end of the right
+// cell), split in two where the row wraps.
 template <typename F>
 __device__ __forceinline__ void for_neighbors(const NArgs &a, int p, F &&f) {
     const uint32_t c = a.sk[p];
@@ -546,7 +546,7 @@ size_t sort_ws_words(int n, size_t ncells) {
     sums = std::max<size_t>(sums, nblocks_for(ncells));
     return 3 * (size_t)n + 2 * hist + ((size_t)1 << kMaxRadixBits) + sums + 64;
 }
-SortWs sort_ws(void *ws, int n) {  // ws: 8-byte aligned
+SortWs sort_ws(void *ws, int n) {  // ws: 4-byte aligned
     uint32_t *p = static_cast<uint32_t *>(ws);
     SortWs s;
     s.k0 = p;
@@ -652,9 +652,9 @@ int pfs_ngrid_build(const pfs_ngrid *g, const float *x, const float *y, uint32_t
     if (!valid_grid(g)) return set_error(PFS_ERR_INVALID, "ngrid: not initialised by pfs_ngrid_init");
     if (!x || !y || !keys || !order || !cell_start || !cell_end || !xs || !ys)
         return set_error(PFS_ERR_INVALID, "ngrid: NULL buffer");
-    if (!ws || ws_bytes < pfs_ngrid_workspace_bytes(g) || ((uintptr_t)ws & 7))
+    if (!ws || ws_bytes < pfs_ngrid_workspace_bytes(g) || ((uintptr_t)ws & 3))
         return set_error(PFS_ERR_INVALID,
-                         "ngrid: workspace not 8-byte aligned or smaller than pfs_ngrid_workspace_bytes()");
+                         "ngrid: workspace not 4-byte aligned or smaller than pfs_ngrid_workspace_bytes()");
     hipStream_t s = (hipStream_t)stream;
     const int n = g->n, nb = nblocks_for((size_t)n);
     const uint32_t ncells = (uint32_t)g->gx * (uint32_t)g->gy;
@@ -717,9 +717,9 @@ int pfs_ngrid_neighbors(const pfs_ngrid *g, const float *xs, const float *ys, co
     if (!valid_grid(g)) return set_error(PFS_ERR_INVALID, "ngrid: not initialised by pfs_ngrid_init");
     if (!xs || !ys || !cell_start || !cell_end || !counts || !offsets || !total)
         return set_error(PFS_ERR_INVALID, "ngrid: NULL buffer");
-    if (!ws || ws_bytes < pfs_ngrid_workspace_bytes(g) || ((uintptr_t)ws & 7))
+    if (!ws || ws_bytes < pfs_ngrid_workspace_bytes(g) || ((uintptr_t)ws & 3))
         return set_error(PFS_ERR_INVALID,
-                         "ngrid: workspace not 8-byte aligned or smaller than pfs_ngrid_workspace_bytes()");
+                         "ngrid: workspace not 4-byte aligned or smaller than pfs_ngrid_workspace_bytes()");
     hipStream_t s = (hipStream_t)stream;
     SortWs w = sort_ws(ws, g->n);
     const NArgs a = nargs(g, xs, ys, order, w.k0, cell_start, cell_end);
@@ -744,9 +744,9 @@ int pfs_ngrid_density(const pfs_ngrid *g, const float *xs, const float *ys, cons
     if (!valid_grid(g)) return set_error(PFS_ERR_INVALID, "ngrid: not initialised by pfs_ngrid_init");
     if (!xs || !ys || !cell_start || !cell_end || !rho)
         return set_error(PFS_ERR_INVALID, "ngrid: NULL buffer");
-    if (!ws || ws_bytes < pfs_ngrid_workspace_bytes(g) || ((uintptr_t)ws & 7))
+    if (!ws || ws_bytes < pfs_ngrid_workspace_bytes(g) || ((uintptr_t)ws & 3))
         return set_error(PFS_ERR_INVALID,
-                         "ngrid: workspace not 8-byte aligned or smaller than pfs_ngrid_workspace_bytes()");
+                         "ngrid: workspace not 4-byte aligned or smaller than pfs_ngrid_workspace_bytes()");
     SortWs w = sort_ws(ws, g->n);
     hipStream_t s = (hipStream_t)stream;
     const NArgs a = nargs(g, xs, ys, order, w.k0, cell_start, cell_end);
