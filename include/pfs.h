@@ -130,6 +130,28 @@ int pfs_advect_color_step_dev(float **image, float **itmp, float **vp, float dt,
 int pfs_set_step_path(int path);
 int pfs_get_step_path(void);
 
+/* Tile side of the fused step's workgroups (process-wide).  No counterpart
+ * in the reference, whose CUDA kernels use fixed 16x16 blocks
+ * (src/fluid.cu:14-15).  Each workgroup of the fused kernels owns a side x
+ * side tile of cells; results do not depend on the side.  Unless forced
+ * here, a step takes the side pfs_autotune_step measured fastest for its
+ * shape (field size, iters, dye fused or not), else the largest side whose
+ * tiles fill every CU (a small field: 16).  side 0 clears a forced side. */
+int pfs_set_step_tile(int side);
+/* The side a step of this shape takes now (16, 32 or 64), or < 0 on error. */
+int pfs_get_step_tile(int vx, int vy, int iters, int dye_fused);
+/* Times pfs_fluid_timestep (image/itmp NULL: pfs_simulate_fluid_step) on
+ * scratch copies of the buffers, `reps` steps per side after one untimed
+ * step, for the sides 64, 32 and 16 in that order (ms[0..2], per step, if
+ * ms is non-NULL with n_ms >= 3), and records the fastest for this shape:
+ * later steps of the shape take it.  The caller's buffers are not changed.
+ * Allocates the scratch copies; synchronises the stream. */
+int pfs_autotune_step(const pfs_field *vp, const pfs_field *tmp, const pfs_field *image,
+                      const pfs_field *itmp, float dt, float viscosity, int iters, int reps,
+                      void *stream, int *side, float *ms, int n_ms);
+/* Forgets every side pfs_autotune_step recorded. */
+int pfs_clear_step_tuning(void);
+
 /* Device scratch of the fused step: eight float planes of vx*vy cells (the
  * pressure start, the divergence, and two iterate planes each for diffusion
  * and pressure when iters exceeds one launch's sweeps).  The library keeps
@@ -183,6 +205,19 @@ size_t pfs_batch_workspace_bytes(pfs_batch *b);
 /* 1 (default): replay steps from a hipGraph; 0: launch every kernel. */
 int pfs_batch_set_graphs(pfs_batch *b, int enable);
 void *pfs_batch_stream(pfs_batch *b);
+/* Tile side of the batch's fused kernels (one side for every scene of a
+ * launch; results do not depend on it).  side 0 (the default): the largest
+ * side with a tile for every two CUs over all live scenes, else 16, chosen
+ * again whenever membership changes; else the forced side. */
+int pfs_batch_set_tile(pfs_batch *b, int side);
+/* The side the next step uses (16, 32 or 64), or < 0 on error. */
+int pfs_batch_tile(pfs_batch *b);
+/* Times 2*reps steps of the batch for the sides 64, 32 and 16 (ms[0..2] per
+ * step if ms is non-NULL with n_ms >= 3) and forces the fastest on the batch
+ * (pfs_batch_set_tile(b, 0) returns to the automatic choice).  The scenes end
+ * as they began: their buffers are snapshotted into a device allocation
+ * of their size and copied back after each side.  Synchronises the batch. */
+int pfs_batch_autotune(pfs_batch *b, int reps, int *side, float *ms, int n_ms);
 
 /* ---- precision form --------------------------------------------------------
  * No counterpart in the reference, whose Jacobi loops run a fixed count in

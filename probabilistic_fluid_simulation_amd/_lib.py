@@ -73,6 +73,11 @@ SIGNATURES = {
     "pfs_simulate_fluid_step_dev": [_FPP, _FPP, _F, _F, _I, _I, _I, _I, _V],
     "pfs_advect_color_step_dev": [_FPP, _FPP, _FPP, _F, _I, _I, _I, _I, _I, _I, _V],
     "pfs_set_step_path": [_I],
+    "pfs_set_step_tile": [_I],
+    "pfs_get_step_tile": [_I, _I, _I, _I],
+    "pfs_autotune_step": [_P, _P, _P, _P, _F, _F, _I, _I, _V, ctypes.POINTER(ctypes.c_int),
+                          ctypes.POINTER(ctypes.c_float), _I],
+    "pfs_clear_step_tuning": [],
     "pfs_get_step_path": [],
     "pfs_step_workspace_bytes": [_I, _I],
     "pfs_profile_step": [_P, _P, _P, _P, _F, _F, _I, _I, _V, ctypes.POINTER(ctypes.c_float), _I],
@@ -89,6 +94,10 @@ SIGNATURES = {
     "pfs_batch_workspace_bytes": [_V],
     "pfs_batch_set_graphs": [_V, _I],
     "pfs_batch_stream": [_V],
+    "pfs_batch_set_tile": [_V, _I],
+    "pfs_batch_tile": [_V],
+    "pfs_batch_autotune": [_V, _I, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float),
+                           _I],
     "pfs_precise_workspace_bytes": [_I, _I, _I],
     "pfs_simulate_fluid_step_precise": [_P, _P, _F, _F, _I, ctypes.c_double, _V, ctypes.c_size_t,
                                         _V, ctypes.POINTER(PreciseReport)],

@@ -98,6 +98,26 @@ class Batch:
     def workspace_bytes(self) -> int:
         return int(self._lib.pfs_batch_workspace_bytes(self._h))
 
+    def set_tile(self, side: int) -> None:
+        """Force the fused kernels' tile side (16, 32, 64); 0: automatic."""
+        _lib.call("pfs_batch_set_tile", self._h, side)
+
+    @property
+    def tile(self) -> int:
+        """The tile side the next step uses."""
+        side = int(self._lib.pfs_batch_tile(self._h))
+        if side < 0:
+            raise _lib.PfsError("pfs_batch_tile", side, self._lib.pfs_last_error().decode())
+        return side
+
+    def autotune(self, reps: int = 10):
+        """Time every tile side on the live scenes (their state is restored)
+        and force the fastest; returns (side, {side: ms per step})."""
+        ms = (ctypes.c_float * 3)()
+        side = ctypes.c_int(0)
+        _lib.call("pfs_batch_autotune", self._h, reps, ctypes.byref(side), ms, 3)
+        return side.value, {t: float(m) for t, m in zip((64, 32, 16), ms)}
+
     @property
     def stream_handle(self) -> Optional[int]:
         return self._lib.pfs_batch_stream(self._h)

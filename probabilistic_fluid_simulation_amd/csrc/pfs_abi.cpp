@@ -9,6 +9,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <utility>
 
 #include "pfs.h"
@@ -57,6 +58,56 @@ int get_workspace(size_t bytes, hipStream_t s, float **out) {
 
 bool same_shape(const pfs_field *a, const pfs_field *b) {
     return a->x == b->x && a->y == b->y && a->z == b->z;
+}
+
+// Tile side of the fused step: the side pfs_set_step_tile forced, else the
+// side pfs_autotune_step measured fastest for this shape, else the rule.
+int g_step_tile = 0;  // 0: not forced
+std::mutex g_tune_mutex;
+std::map<std::tuple<int, int, int, int>, int> g_tuned;  // (w, h, iters, dye fused) -> side
+
+}  // namespace
+
+int device_cus() {
+    static std::mutex m;
+    static std::map<int, int> cus;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    std::lock_guard<std::mutex> lock(m);
+    auto it = cus.find(dev);
+    if (it != cus.end()) return it->second;
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n < 1)
+        n = 256;
+    cus[dev] = n;
+    return n;
+}
+
+// The rule: the largest side with at least one tile for every two CUs; a
+// field too small for that takes the smallest side, whose workgroups each
+// hold the fewest rows per thread.  (Measured on MI355X, 256 CUs: 64-wide
+// tiles win from 144 tiles up (768^2), 32-wide ones at 512^2, 16-wide ones
+// at 256^2 and below.)
+int tile_for_counts(long long tiles64, long long tiles32) {
+    const long long half = device_cus() / 2;
+    return tiles64 >= half ? 64 : tiles32 >= half ? 32 : 16;
+}
+
+long long tile_count(int w, int h, int ts) {
+    return (long long)((w + ts - 1) / ts) * ((h + ts - 1) / ts);
+}
+
+namespace {
+int rule_tile(int w, int h) { return tile_for_counts(tile_count(w, h, 64), tile_count(w, h, 32)); }
+
+int step_tile(int w, int h, int iters, bool dye_fused) {
+    if (g_step_tile) return g_step_tile;
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        auto it = g_tuned.find({w, h, iters, dye_fused ? 1 : 0});
+        if (it != g_tuned.end()) return it->second;
+    }
+    return rule_tile(w, h);
 }
 }  // namespace
 
@@ -198,7 +249,8 @@ static bool fused_ok(const pfs_field *vp, const pfs_field *tmp, int iters) {
 // (image, itmp) as one timestep of src/main.cpp:222-225.  `ev` (4 events)
 // requests per-kernel timing of the fused path.
 static int timestep(pfs_field *vp, pfs_field *tmp, pfs_field *image, pfs_field *itmp, float dt,
-                    float viscosity, int iters, void *stream, hipEvent_t *ev, int reps = 1) {
+                    float viscosity, int iters, void *stream, hipEvent_t *ev, int reps = 1,
+                    int tile = 0) {
     PFS_TRY(check_step_fields(vp, tmp, iters));
     if (image) {
         PFS_TRY(check_field(image, "image", 1));
@@ -221,7 +273,8 @@ static int timestep(pfs_field *vp, pfs_field *tmp, pfs_field *image, pfs_field *
     float *ws = nullptr;
     PFS_TRY(get_workspace(fused_workspace_bytes(vp->x, vp->y), (hipStream_t)stream, &ws));
     StepArgs a{vp->data, tmp->data, dye_fused ? image->data : nullptr,
-               dye_fused ? itmp->data : nullptr, vp->x, vp->y, dt, viscosity, iters};
+               dye_fused ? itmp->data : nullptr, vp->x, vp->y, dt, viscosity, iters,
+               tile ? tile : step_tile(vp->x, vp->y, iters, dye_fused)};
     PFS_TRY(fused_step(a, ws, (hipStream_t)stream, ev, reps));
     if (!image) return PFS_OK;
     if (!dye_fused) return pfs_advect_color_step(image, itmp, vp, dt, stream);
@@ -275,6 +328,111 @@ int pfs_set_step_path(int path) {
 }
 
 int pfs_get_step_path(void) { return g_step_path; }
+
+int pfs_set_step_tile(int side) {
+    if (side != 0 && !valid_tile_side(side))
+        return set_error(PFS_ERR_INVALID, "tile side must be 0 (automatic), 16, 32 or 64");
+    g_step_tile = side;
+    return PFS_OK;
+}
+
+int pfs_get_step_tile(int vx, int vy, int iters, int dye_fused) {
+    if (vx < 1 || vy < 1 || iters < 1) return set_error(PFS_ERR_INVALID, "bad step shape");
+    return step_tile(vx, vy, iters, dye_fused != 0);
+}
+
+int pfs_autotune_step(const pfs_field *vp, const pfs_field *tmp, const pfs_field *image,
+                      const pfs_field *itmp, float dt, float viscosity, int iters, int reps,
+                      void *stream, int *side, float *ms, int n_ms) {
+    PFS_TRY(check_step_fields(vp, tmp, iters));
+    if (reps < 1 || !side) return set_error(PFS_ERR_INVALID, "autotune: reps >= 1 and side needed");
+    if (ms && n_ms < 3) return set_error(PFS_ERR_INVALID, "autotune: ms needs room for 3 values");
+    if ((image == nullptr) != (itmp == nullptr))
+        return set_error(PFS_ERR_INVALID, "image and itmp must both be given or both be NULL");
+    if (image) {
+        PFS_TRY(check_field(image, "image", 1));
+        if (!same_shape(image, itmp)) return set_error(PFS_ERR_INVALID, "itmp shape != image shape");
+    }
+    if (!fused_ok(vp, tmp, iters))
+        return set_error(PFS_ERR_UNSUPPORTED, "autotune: the step does not take the fused path");
+    hipStream_t s = (hipStream_t)stream;
+    // Scratch copies: the caller's buffers are read, never advanced.
+    const size_t fb = (size_t)vp->x * vp->y * vp->z * sizeof(float);
+    const size_t ib = image ? (size_t)image->x * image->y * image->z * sizeof(float) : 0;
+    char *scratch = nullptr;
+    const size_t fb_al = (fb + 255) & ~(size_t)255, ib_al = (ib + 255) & ~(size_t)255;
+    if (hipMalloc((void **)&scratch, 2 * fb_al + 2 * ib_al) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(PFS_ERR_NOMEM, "autotune: scratch allocation failed");
+    }
+    pfs_field v{vp->x, vp->y, vp->z, reinterpret_cast<float *>(scratch)};
+    pfs_field t{vp->x, vp->y, vp->z, reinterpret_cast<float *>(scratch + fb_al)};
+    pfs_field im{}, it{};
+    if (image) {
+        im = pfs_field{image->x, image->y, image->z, reinterpret_cast<float *>(scratch + 2 * fb_al)};
+        it = pfs_field{image->x, image->y, image->z,
+                       reinterpret_cast<float *>(scratch + 2 * fb_al + ib_al)};
+    }
+    hipEvent_t ev[2] = {};
+    int rc = PFS_OK;
+    float best_ms = 0.0f;
+    int best = 0;
+    // The shape key as the caller's own step would see it (timestep's test).
+    const bool dye_fused = image && image->x == vp->x && image->y == vp->y && image->z == 4 &&
+                           aligned16(image->data) && aligned16(itmp->data);
+    for (int k = 0; k < 2 && rc == PFS_OK; ++k)
+        if (hipEventCreate(&ev[k]) != hipSuccess) rc = set_error(PFS_ERR_HIP, "autotune: event");
+    for (int k = 0; k < 3 && rc == PFS_OK; ++k) {
+        const int ts = kTileSides[k];
+        hipError_t e = hipMemcpyAsync(v.data, vp->data, fb, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(t.data, tmp->data, fb, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess && image)
+            e = hipMemcpyAsync(im.data, image->data, ib, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) {
+            rc = set_error(PFS_ERR_HIP, std::string("autotune copy: ") + hipGetErrorString(e));
+            break;
+        }
+        // One untimed step (first launches of these kernels), then reps timed.
+        for (int r = 0; r <= reps && rc == PFS_OK; ++r) {
+            if (r == 1 && hipEventRecord(ev[0], s) != hipSuccess) rc = set_error(PFS_ERR_HIP, "autotune: event");
+            if (rc == PFS_OK)
+                rc = timestep(&v, &t, image ? &im : nullptr, image ? &it : nullptr, dt, viscosity,
+                              iters, stream, nullptr, 1, ts);
+        }
+        if (rc != PFS_OK) break;
+        float el = 0.0f;
+        e = hipEventRecord(ev[1], s);
+        if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+        if (e == hipSuccess) e = hipEventElapsedTime(&el, ev[0], ev[1]);
+        if (e != hipSuccess) {
+            rc = set_error(PFS_ERR_HIP, std::string("autotune timing: ") + hipGetErrorString(e));
+            break;
+        }
+        el /= (float)reps;
+        if (ms) ms[k] = el;
+        if (best == 0 || el < best_ms) {
+            best = ts;
+            best_ms = el;
+        }
+    }
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(scratch);
+    if (rc != PFS_OK) return rc;
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        g_tuned[{vp->x, vp->y, iters, dye_fused ? 1 : 0}] = best;
+    }
+    *side = best;
+    return PFS_OK;
+}
+
+int pfs_clear_step_tuning(void) {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    g_tuned.clear();
+    return PFS_OK;
+}
 
 size_t pfs_step_workspace_bytes(int vx, int vy) {
     if (vx < 1 || vy < 1) return 0;

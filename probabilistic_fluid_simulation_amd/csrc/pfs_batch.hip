@@ -36,21 +36,20 @@ namespace {
 
 constexpr int kCBX = 64, kCBY = 4;  // dye blocks: 64 x 4 pixels
 
-template <int H, bool LAST>
-__global__ __launch_bounds__(Geo<H>::NT) void k_diffuse_batch(const DiffuseArgs *__restrict__ tab,
-                                                              const int *__restrict__ tile_scene,
-                                                              int ntiles) {
+template <int H, bool LAST, int TS>
+__global__ __launch_bounds__((Geo<H, TS>::NT)) void k_diffuse_batch(
+    const DiffuseArgs *__restrict__ tab, const int *__restrict__ tile_scene, int ntiles) {
     const int b = xcd_tile(blockIdx.x, ntiles);
     const DiffuseArgs a = tab[tile_scene[b]];
-    diffuse_tile<H, LAST>(a, b - a.tile_base);
+    diffuse_tile<H, LAST, TS>(a, b - a.tile_base);
 }
 
-template <int H, bool LAST>
-__global__ __launch_bounds__(Geo<H>::NT) void k_pressure_batch(
+template <int H, bool LAST, int TS>
+__global__ __launch_bounds__((Geo<H, TS>::NT)) void k_pressure_batch(
     const PressureArgs *__restrict__ tab, const int *__restrict__ tile_scene, int ntiles) {
     const int b = xcd_tile(blockIdx.x, ntiles);
     const PressureArgs a = tab[tile_scene[b]];
-    pressure_tile<H, LAST>(a, b - a.tile_base);
+    pressure_tile<H, LAST, TS>(a, b - a.tile_base);
 }
 
 __global__ __launch_bounds__(kCBX *kCBY) void k_color_batch(const ColorArgs *__restrict__ tab,
@@ -67,26 +66,39 @@ using PressureBatchFn = void (*)(const PressureArgs *, const int *, int);
 
 #define PFS_H_CASES(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9)
 
-DiffuseBatchFn diffuse_batch_for(int H, bool last) {
+template <int TS>
+DiffuseBatchFn diffuse_batch_for_ts(int H, bool last) {
     switch (H) {
 #define PFS_CASE(h) \
     case h:         \
-        return last ? k_diffuse_batch<h, true> : k_diffuse_batch<h, false>;
+        return last ? k_diffuse_batch<h, true, TS> : k_diffuse_batch<h, false, TS>;
         PFS_H_CASES(PFS_CASE)
 #undef PFS_CASE
     }
     return nullptr;
 }
 
-PressureBatchFn pressure_batch_for(int H, bool last) {
+template <int TS>
+PressureBatchFn pressure_batch_for_ts(int H, bool last) {
     switch (H) {
 #define PFS_CASE(h) \
     case h:         \
-        return last ? k_pressure_batch<h, true> : k_pressure_batch<h, false>;
+        return last ? k_pressure_batch<h, true, TS> : k_pressure_batch<h, false, TS>;
         PFS_H_CASES(PFS_CASE)
 #undef PFS_CASE
     }
     return nullptr;
+}
+
+DiffuseBatchFn diffuse_batch_for(int H, bool last, int ts) {
+    return ts == 16 ? diffuse_batch_for_ts<16>(H, last)
+                    : ts == 32 ? diffuse_batch_for_ts<32>(H, last) : diffuse_batch_for_ts<64>(H, last);
+}
+
+PressureBatchFn pressure_batch_for(int H, bool last, int ts) {
+    return ts == 16 ? pressure_batch_for_ts<16>(H, last)
+                    : ts == 32 ? pressure_batch_for_ts<32>(H, last)
+                               : pressure_batch_for_ts<64>(H, last);
 }
 
 constexpr int kMaxPlan = 64;  // chunks of one step: K up to 512
@@ -132,6 +144,8 @@ struct pfs_batch {
     std::vector<Scene> scenes;
     bool dirty = true;
     bool use_graph = true;
+    int tile_forced = 0;  // 0: the rule over the live scenes' tile counts
+    int tile = kTile;     // tile side of the current tables
     std::vector<Group> groups;
     ColorArgs *ctab[2] = {};
     int *block_scene = nullptr;
@@ -191,6 +205,17 @@ struct pfs_batch {
         return PFS_OK;
     }
 
+    // The rule over every live scene's tiles (one launch covers them all).
+    int auto_tile() const {
+        long long n64 = 0, n32 = 0;
+        for (const Scene &s : scenes)
+            if (s.live) {
+                n64 += tile_count(s.vp.x, s.vp.y, 64);
+                n32 += tile_count(s.vp.x, s.vp.y, 32);
+            }
+        return tile_for_counts(n64, n32);
+    }
+
     // Rebuilds every table for the live scenes.  Table parity q is the
     // parity of (t - t_build): scene s then reads its image from
     // dye[(cur0 + t_build - t0 + q) & 1].
@@ -198,6 +223,7 @@ struct pfs_batch {
         PFS_TRY(reap(false));
         PFS_TRY(drop_tables());
         t_build = t;
+        tile = tile_forced ? tile_forced : auto_tile();
         std::map<int, std::vector<int>> by_k;
         std::vector<int> unfused;
         for (int i = 0; i < (int)scenes.size(); ++i) {
@@ -228,10 +254,10 @@ struct pfs_batch {
             for (size_t k = 0; k < ids.size(); ++k) {
                 const Scene &s = scenes[ids[k]];
                 d[k] = diffuse_args(s.vp.data, s.tmp.data, s.vp.x, s.vp.y, s.dt, s.visc, s.iters,
-                                    s.ws);
+                                    s.ws, tile);
                 d[k].tile_base = (int)tile_scene.size();
                 p[k] = pressure_args(s.vp.data, s.tmp.data, nullptr, nullptr, s.vp.x, s.vp.y, s.dt,
-                                     s.ws);
+                                     s.ws, tile);
                 p[k].tile_base = d[k].tile_base;
                 tile_scene.insert(tile_scene.end(), d[k].ntiles, (int)k);
             }
@@ -320,14 +346,14 @@ struct pfs_batch {
     int enqueue_step(int q) {
         for (const Group &g : groups) {
             for (int c = 0; c < g.nchunks; ++c) {
-                hipLaunchKernelGGL(diffuse_batch_for(g.plan[c].H, g.plan[c].last), dim3(g.ntiles),
-                                   dim3(threads_for(g.plan[c].H)), 0, stream, g.dtab[c],
-                                   g.tile_scene, g.ntiles);
+                hipLaunchKernelGGL(diffuse_batch_for(g.plan[c].H, g.plan[c].last, tile),
+                                   dim3(g.ntiles), dim3(threads_for(g.plan[c].H, tile)), 0, stream,
+                                   g.dtab[c], g.tile_scene, g.ntiles);
                 PFS_TRY(check_launch("batched diffuse"));
             }
             for (int c = 0; c < g.nchunks; ++c) {
-                hipLaunchKernelGGL(pressure_batch_for(g.plan[c].H, g.plan[c].last),
-                                   dim3(g.ntiles), dim3(threads_for(g.plan[c].H)), 0, stream,
+                hipLaunchKernelGGL(pressure_batch_for(g.plan[c].H, g.plan[c].last, tile),
+                                   dim3(g.ntiles), dim3(threads_for(g.plan[c].H, tile)), 0, stream,
                                    g.ptab[q][c], g.tile_scene, g.ntiles);
                 PFS_TRY(check_launch("batched pressure"));
             }
@@ -519,5 +545,103 @@ int pfs_batch_set_graphs(pfs_batch *b, int enable) {
 }
 
 void *pfs_batch_stream(pfs_batch *b) { return b ? (void *)b->stream : nullptr; }
+
+int pfs_batch_set_tile(pfs_batch *b, int side) {
+    if (!b) return set_error(PFS_ERR_INVALID, "batch is NULL");
+    if (side != 0 && !valid_tile_side(side))
+        return set_error(PFS_ERR_INVALID, "tile side must be 0 (automatic), 16, 32 or 64");
+    if (side != b->tile_forced) {
+        b->tile_forced = side;
+        b->dirty = true;
+    }
+    return PFS_OK;
+}
+
+int pfs_batch_tile(pfs_batch *b) {
+    if (!b) return set_error(PFS_ERR_INVALID, "batch is NULL");
+    return b->tile_forced ? b->tile_forced : b->auto_tile();
+}
+
+int pfs_batch_autotune(pfs_batch *b, int reps, int *side, float *ms, int n_ms) {
+    if (!b || !side || reps < 1) return set_error(PFS_ERR_INVALID, "autotune: batch, side, reps >= 1");
+    if (ms && n_ms < 3) return set_error(PFS_ERR_INVALID, "autotune: ms needs room for 3 values");
+    if (pfs_batch_count(b) == 0) return set_error(PFS_ERR_INVALID, "autotune: the batch is empty");
+    PFS_TRY(b->sync());
+    // Snapshot every live scene's four buffers; the timed steps run on the
+    // scenes themselves and the snapshot is copied back after each side.
+    struct Buf {
+        float *dst;
+        size_t bytes, off;
+    };
+    std::vector<Buf> bufs;
+    size_t total = 0;
+    for (auto &sc : b->scenes) {
+        if (!sc.live) continue;
+        for (const pfs_field *f : {&sc.vp, &sc.tmp, &sc.dye[0], &sc.dye[1]}) {
+            const size_t bytes = (size_t)f->x * f->y * f->z * sizeof(float);
+            bufs.push_back({f->data, bytes, total});
+            total += (bytes + 255) & ~(size_t)255;
+        }
+    }
+    char *snap = nullptr;
+    if (hipMalloc((void **)&snap, total) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(PFS_ERR_NOMEM, "autotune: snapshot allocation failed");
+    }
+    auto copy_all = [&](bool save) -> int {
+        for (const Buf &x : bufs) {
+            hipError_t e = save ? hipMemcpyAsync(snap + x.off, x.dst, x.bytes, hipMemcpyDeviceToDevice, b->stream)
+                                : hipMemcpyAsync(x.dst, snap + x.off, x.bytes, hipMemcpyDeviceToDevice, b->stream);
+            if (e != hipSuccess)
+                return set_error(PFS_ERR_HIP, std::string("autotune copy: ") + hipGetErrorString(e));
+        }
+        return PFS_OK;
+    };
+    const long long t_saved = b->t;
+    const int forced_saved = b->tile_forced;
+    hipEvent_t ev[2] = {};
+    int rc = copy_all(true);
+    for (int k = 0; k < 2 && rc == PFS_OK; ++k)
+        if (hipEventCreate(&ev[k]) != hipSuccess) rc = set_error(PFS_ERR_HIP, "autotune: event");
+    int best = 0;
+    float best_ms = 0.0f;
+    for (int k = 0; k < 3 && rc == PFS_OK; ++k) {
+        b->tile_forced = kTileSides[k];
+        b->dirty = true;
+        rc = b->step(2);  // untimed: tables, graph, first launches
+        if (rc == PFS_OK && hipEventRecord(ev[0], b->stream) != hipSuccess)
+            rc = set_error(PFS_ERR_HIP, "autotune: event");
+        if (rc == PFS_OK) rc = b->step(2 * reps);
+        float el = 0.0f;
+        if (rc == PFS_OK) {
+            hipError_t e = hipEventRecord(ev[1], b->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+            if (e == hipSuccess) e = hipEventElapsedTime(&el, ev[0], ev[1]);
+            if (e != hipSuccess)
+                rc = set_error(PFS_ERR_HIP, std::string("autotune timing: ") + hipGetErrorString(e));
+        }
+        const int rc2 = copy_all(false);  // back to the state before tuning
+        b->t = t_saved;
+        b->dirty = true;
+        if (rc == PFS_OK) rc = rc2;
+        if (rc != PFS_OK) break;
+        el /= (float)(2 * reps);
+        if (ms) ms[k] = el;
+        if (best == 0 || el < best_ms) {
+            best = kTileSides[k];
+            best_ms = el;
+        }
+    }
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    const int rcs = b->sync();
+    (void)hipFree(snap);
+    b->tile_forced = rc == PFS_OK ? best : forced_saved;
+    b->dirty = true;
+    if (rc != PFS_OK) return rc;
+    if (rcs != PFS_OK) return rcs;
+    *side = best;
+    return PFS_OK;
+}
 
 }  // extern "C"

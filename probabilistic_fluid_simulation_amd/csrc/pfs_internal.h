@@ -73,6 +73,17 @@ int launch_advect_color_keys(const float *vp, int *keys, int iw, int ih, int vw,
                              float dt, hipStream_t s);
 
 // ---- fused step (pfs_step.hip) -------------------------------------------
+// Tile sides the fused kernels are built for.  A small field has few tiles
+// of the largest side, leaving most CUs idle; smaller tiles spread it over
+// more workgroups, each with fewer rows per thread, at the price of a
+// relatively wider halo.
+constexpr int kTileSides[3] = {64, 32, 16};
+inline bool valid_tile_side(int ts) { return ts == 64 || ts == 32 || ts == 16; }
+int device_cus();  // CUs of the current device (256 if it does not answer)
+// The tile side for a launch of this many tiles of side 64 and 32.
+int tile_for_counts(long long tiles64, long long tiles32);
+long long tile_count(int w, int h, int ts);
+
 struct StepArgs {
     float *P;            // vp  (z == 4), device
     float *Q;            // tmp (z == 4), device
@@ -81,6 +92,7 @@ struct StepArgs {
     int w, h;
     float dt, visc;
     int iters;
+    int tile;  // tile side of the fused kernels: 64, 32 or 16
 };
 // `ws` is a device workspace of fused_workspace_bytes(w, h).  Optional
 // per-kernel event timing: if `ev` is non-null, events ev[2k] and ev[2k+1]
@@ -90,7 +102,7 @@ struct StepArgs {
 // idempotent: neither reads what it writes).
 int fused_step(const StepArgs &a, float *ws, hipStream_t s, hipEvent_t *ev, int reps = 1);
 size_t fused_workspace_bytes(int w, int h);
-int threads_for(int H);  // threads of the fused kernels for halo H
+int threads_for(int H, int ts = 64);  // threads of the fused kernels for halo H, tile side ts
 
 // Validation helpers.
 int check_field(const pfs_field *f, const char *name, int min_channels);

@@ -10,7 +10,8 @@
 //      of the same cells (:72-127) when one call runs the whole timestep of
 //      the reference driver (src/main.cpp:222-225).
 //
-// Each workgroup owns a TX x TY tile of cells and stages the tile plus a
+// Each workgroup owns a TX x TY tile of cells (64, 32 or 16 square: the
+// caller's choice) and stages the tile plus a
 // halo of H = sweeps (+1 for the final stencil) cells, periodic wrap folded
 // into the staged coordinates.  Thread (x, seg) owns column x of the staged
 // region, rows [seg*S, seg*S + S), in registers.  Every sweep it publishes
@@ -42,55 +43,72 @@ namespace pfs {
 
 namespace {
 
-template <int H, bool LAST>
-__global__ __launch_bounds__(Geo<H>::NT) void k_diffuse(DiffuseArgs a) {
-    diffuse_tile<H, LAST>(a, xcd_tile(blockIdx.x, a.ntiles));
+template <int H, bool LAST, int TS>
+__global__ __launch_bounds__((Geo<H, TS>::NT)) void k_diffuse(DiffuseArgs a) {
+    diffuse_tile<H, LAST, TS>(a, xcd_tile(blockIdx.x, a.ntiles));
 }
 
-template <int H, bool LAST>
-__global__ __launch_bounds__(Geo<H>::NT) void k_pressure(PressureArgs a) {
-    pressure_tile<H, LAST>(a, xcd_tile(blockIdx.x, a.ntiles));
+template <int H, bool LAST, int TS>
+__global__ __launch_bounds__((Geo<H, TS>::NT)) void k_pressure(PressureArgs a) {
+    pressure_tile<H, LAST, TS>(a, xcd_tile(blockIdx.x, a.ntiles));
 }
 
-// ---- host dispatch over the compile-time halo ------------------------------
+// ---- host dispatch over the compile-time halo and tile side ----------------
 using DiffuseFn = void (*)(DiffuseArgs);
 using PressureFn = void (*)(PressureArgs);
 
 #define PFS_H_CASES(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9)
 
-DiffuseFn diffuse_for(int H, bool last) {
+template <int TS>
+DiffuseFn diffuse_for_ts(int H, bool last) {
     switch (H) {
 #define PFS_CASE(h) \
     case h:         \
-        return last ? k_diffuse<h, true> : k_diffuse<h, false>;
+        return last ? k_diffuse<h, true, TS> : k_diffuse<h, false, TS>;
         PFS_H_CASES(PFS_CASE)
 #undef PFS_CASE
     }
     return nullptr;
 }
 
-PressureFn pressure_for(int H, bool last) {
+template <int TS>
+PressureFn pressure_for_ts(int H, bool last) {
     switch (H) {
 #define PFS_CASE(h) \
     case h:         \
-        return last ? k_pressure<h, true> : k_pressure<h, false>;
+        return last ? k_pressure<h, true, TS> : k_pressure<h, false, TS>;
         PFS_H_CASES(PFS_CASE)
 #undef PFS_CASE
     }
     return nullptr;
 }
 
-}  // namespace
+DiffuseFn diffuse_for(int H, bool last, int ts) {
+    return ts == 16 ? diffuse_for_ts<16>(H, last)
+                    : ts == 32 ? diffuse_for_ts<32>(H, last) : diffuse_for_ts<64>(H, last);
+}
 
-int threads_for(int H) {
+PressureFn pressure_for(int H, bool last, int ts) {
+    return ts == 16 ? pressure_for_ts<16>(H, last)
+                    : ts == 32 ? pressure_for_ts<32>(H, last) : pressure_for_ts<64>(H, last);
+}
+
+template <int TS>
+int threads_for_ts(int H) {
     switch (H) {
 #define PFS_CASE(h) \
     case h:         \
-        return Geo<h>::NT;
+        return Geo<h, TS>::NT;
         PFS_H_CASES(PFS_CASE)
 #undef PFS_CASE
     }
     return 0;
+}
+
+}  // namespace
+
+int threads_for(int H, int ts) {
+    return ts == 16 ? threads_for_ts<16>(H) : ts == 32 ? threads_for_ts<32>(H) : threads_for_ts<64>(H);
 }
 
 size_t fused_workspace_bytes(int w, int h) {
@@ -103,8 +121,9 @@ int fused_step(const StepArgs &s, float *ws, hipStream_t stream, hipEvent_t *ev,
     const int nchunks = chunk_plan(s.iters, plan, 64);
     if (plan[nchunks - 1].last == false)
         return set_error(PFS_ERR_UNSUPPORTED, "iters too large for one step plan");
-    DiffuseArgs d = diffuse_args(s.P, s.Q, s.w, s.h, s.dt, s.visc, s.iters, ws);
-    PressureArgs p = pressure_args(s.P, s.Q, s.image, s.itmp, s.w, s.h, s.dt, ws);
+    if (!valid_tile_side(s.tile)) return set_error(PFS_ERR_INVALID, "tile side must be 16, 32 or 64");
+    DiffuseArgs d = diffuse_args(s.P, s.Q, s.w, s.h, s.dt, s.visc, s.iters, ws, s.tile);
+    PressureArgs p = pressure_args(s.P, s.Q, s.image, s.itmp, s.w, s.h, s.dt, ws, s.tile);
     const dim3 grid(d.ntiles);
 
     if (!ev) reps = 1;
@@ -112,8 +131,8 @@ int fused_step(const StepArgs &s, float *ws, hipStream_t stream, hipEvent_t *ev,
     for (int r = 0; r < reps; ++r)
         for (int c = 0; c < nchunks; ++c) {
             diffuse_chunk(d, c, plan[c], ws);
-            hipLaunchKernelGGL(diffuse_for(plan[c].H, plan[c].last), grid,
-                               dim3(threads_for(plan[c].H)), 0, stream, d);
+            hipLaunchKernelGGL(diffuse_for(plan[c].H, plan[c].last, s.tile), grid,
+                               dim3(threads_for(plan[c].H, s.tile)), 0, stream, d);
             PFS_TRY(check_launch("fused diffuse"));
         }
     if (ev) PFS_HIP_TRY(hipEventRecord(ev[1], stream));
@@ -121,8 +140,8 @@ int fused_step(const StepArgs &s, float *ws, hipStream_t stream, hipEvent_t *ev,
     for (int r = 0; r < reps; ++r)
         for (int c = 0; c < nchunks; ++c) {
             pressure_chunk(p, c, plan[c], ws, s.image, s.itmp);
-            hipLaunchKernelGGL(pressure_for(plan[c].H, plan[c].last), grid,
-                               dim3(threads_for(plan[c].H)), 0, stream, p);
+            hipLaunchKernelGGL(pressure_for(plan[c].H, plan[c].last, s.tile), grid,
+                               dim3(threads_for(plan[c].H, s.tile)), 0, stream, p);
             PFS_TRY(check_launch("fused pressure"));
         }
     if (ev) PFS_HIP_TRY(hipEventRecord(ev[3], stream));

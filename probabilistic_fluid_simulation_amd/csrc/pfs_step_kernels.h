@@ -8,19 +8,24 @@
 
 namespace pfs {
 
-constexpr int kTile = 64;    // tile side
-constexpr int kSegs = 12;    // row segments per staged column
+constexpr int kTile = 64;    // the largest tile side, and the batched form's
+constexpr int kSegs = 12;    // row segments per staged column of a 64-wide tile
 constexpr int kMaxChunk = 8; // sweeps per launch
 
-template <int H_>
+template <int H_, int TS_ = kTile>
 struct Geo {
     static constexpr int H = H_;
-    static constexpr int TX = kTile, TY = kTile;
+    static constexpr int TX = TS_, TY = TS_;
     static constexpr int RX = TX + 2 * H, RY = TY + 2 * H;
-    static constexpr int S = (RY + kSegs - 1) / kSegs;  // rows per thread
-    static constexpr int RYP = kSegs * S;               // rows of an LDS plane (>= RY)
-    static constexpr int NTH = RX * kSegs;              // threads owning cells
+    // Rows per thread: a 64-wide tile keeps 12 segments per column; smaller
+    // tiles take as many segments as 1024 threads allow, rows split evenly.
+    static constexpr int MAXSEG = TS_ == kTile ? kSegs : 1024 / RX;
+    static constexpr int S = (RY + MAXSEG - 1) / MAXSEG;
+    static constexpr int SEGS = TS_ == kTile ? kSegs : (RY + S - 1) / S;
+    static constexpr int RYP = SEGS * S;                // rows of an LDS plane (>= RY)
+    static constexpr int NTH = RX * SEGS;               // threads owning cells
     static constexpr int NT = ((NTH + 63) / 64) * 64;   // whole wavefronts
+    static_assert(NTH <= 1024 && RYP >= RY, "tile geometry");
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
@@ -116,9 +121,9 @@ __device__ __forceinline__ int staged(int g, int o, int n, int lim) {
     return r < lim - 1 ? r : lim;
 }
 
-template <int H, bool LAST>
+template <int H, bool LAST, int TS = kTile>
 __device__ __forceinline__ void diffuse_tile(const DiffuseArgs &a, int tile) {
-    using G = Geo<H>;
+    using G = Geo<H, TS>;
     constexpr int S = G::S, RX = G::RX, RY = G::RY, PL = G::RYP * RX;
     constexpr int NS = H - (LAST ? 1 : 0);  // sweeps in this launch
     __shared__ float2 plane[2 * PL];
@@ -278,9 +283,9 @@ __device__ __forceinline__ void diffuse_tile(const DiffuseArgs &a, int tile) {
     }
 }
 
-template <int H, bool LAST>
+template <int H, bool LAST, int TS = kTile>
 __device__ __forceinline__ void pressure_tile(const PressureArgs &a, int tile) {
-    using G = Geo<H>;
+    using G = Geo<H, TS>;
     constexpr int S = G::S, RX = G::RX, RY = G::RY, PL = G::RYP * RX;
     constexpr int NS = H - (LAST ? 1 : 0);
     __shared__ float plane[2 * PL];
@@ -421,7 +426,7 @@ inline int chunk_plan(int iters, Chunk *out, int max_chunks) {
 
 // Scene-wide arguments of the two kernels (chunk fields set per launch).
 inline DiffuseArgs diffuse_args(const float *P, const float *Q, int w, int h, float dt, float visc,
-                                int iters, float *ws) {
+                                int iters, float *ws, int ts = kTile) {
     DiffuseArgs d{};
     const size_t N = (size_t)w * h;
     d.P = P;
@@ -430,8 +435,8 @@ inline DiffuseArgs diffuse_args(const float *P, const float *Q, int w, int h, fl
     d.bpl = ws + N;
     d.w = w;
     d.h = h;
-    d.tiles_x = (w + kTile - 1) / kTile;
-    d.ntiles = d.tiles_x * ((h + kTile - 1) / kTile);
+    d.tiles_x = (w + ts - 1) / ts;
+    d.ntiles = d.tiles_x * ((h + ts - 1) / ts);
     d.dt = dt;
     d.fw = (float)w;
     d.fh = (float)h;
@@ -466,7 +471,7 @@ inline void diffuse_chunk(DiffuseArgs &d, int c, const Chunk &ch, float *ws) {
 // `image`/`itmp` non-null: advect the dye (same size as the field, 4
 // channels) in the last chunk.
 inline PressureArgs pressure_args(float *P, float *Q, const float *image, float *itmp, int w,
-                                  int h, float dt, float *ws) {
+                                  int h, float dt, float *ws, int ts = kTile) {
     PressureArgs p{};
     const size_t N = (size_t)w * h;
     p.P = P;
@@ -477,8 +482,8 @@ inline PressureArgs pressure_args(float *P, float *Q, const float *image, float 
     p.itmp = itmp;
     p.w = w;
     p.h = h;
-    p.tiles_x = (w + kTile - 1) / kTile;
-    p.ntiles = p.tiles_x * ((h + kTile - 1) / kTile);
+    p.tiles_x = (w + ts - 1) / ts;
+    p.ntiles = p.tiles_x * ((h + ts - 1) / ts);
     p.dt = dt;
     p.fw = (float)w;
     p.fh = (float)h;

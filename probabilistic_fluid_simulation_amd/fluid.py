@@ -224,6 +224,48 @@ def get_step_path() -> str:
         _lib.load().pfs_get_step_path()]
 
 
+TILE_SIDES = (64, 32, 16)
+
+
+def set_step_tile(side: int) -> None:
+    """Force the fused step's tile side (16, 32 or 64); 0 returns to the
+    automatic choice (the autotuner's record for the shape, else the rule)."""
+    _lib.call("pfs_set_step_tile", side)
+
+
+def get_step_tile(w: int, h: int, iters: int, dye_fused: bool = False) -> int:
+    """The tile side a step of this shape takes now."""
+    lib = _lib.load()
+    side = lib.pfs_get_step_tile(w, h, iters, int(bool(dye_fused)))
+    if side < 0:
+        raise _lib.PfsError("pfs_get_step_tile", side, lib.pfs_last_error().decode(errors="replace"))
+    return side
+
+
+def autotune_step(vp: VPField, tmp: VPField, dt: float, viscosity: float, iters: int,
+                  image: Optional[VPField] = None, itmp: Optional[VPField] = None,
+                  reps: int = 20, stream=None):
+    """Time the step (with image/itmp: the whole fluid_timestep) on scratch
+    copies for every tile side and record the fastest for this shape; the
+    buffers are not changed.  Returns (side, {side: ms per step})."""
+    ms = (ctypes.c_float * 3)()
+    side = ctypes.c_int(0)
+    if image is None:
+        b = _Bound(vp, tmp)
+        args = (b.ptr(0), b.ptr(1), None, None)
+    else:
+        b = _Bound(vp, tmp, image, itmp)
+        args = (b.ptr(0), b.ptr(1), b.ptr(2), b.ptr(3))
+    _lib.call("pfs_autotune_step", *args, dt, viscosity, iters, reps, _stream(stream),
+              ctypes.byref(side), ms, 3)
+    return side.value, {t: float(m) for t, m in zip(TILE_SIDES, ms)}
+
+
+def clear_step_tuning() -> None:
+    """Forget every side autotune_step recorded."""
+    _lib.call("pfs_clear_step_tuning")
+
+
 def profile_step(vp: VPField, tmp: VPField, dt: float, viscosity: float, iters: int,
                  image: Optional[VPField] = None, itmp: Optional[VPField] = None, stream=None,
                  reps: int = 1):

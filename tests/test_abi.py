@@ -106,6 +106,32 @@ def test_step_path_switch_and_workspace_size():
     assert lib.pfs_step_workspace_bytes(0, 5) == 0
 
 
+def test_step_tile_rule_and_forcing():
+    """The largest tile side with a tile for every two of the 256 CUs (the
+    default where no device answers), else 16; a forced side wins; bad sides
+    are rejected."""
+    lib = _lib.load()
+    lib.pfs_clear_step_tuning()
+    assert lib.pfs_get_step_tile(1024, 1024, 8, 1) == 64
+    assert lib.pfs_get_step_tile(768, 768, 8, 1) == 64   # 144 tiles of 64
+    assert lib.pfs_get_step_tile(700, 700, 8, 1) == 32   # 11 x 11 = 121 tiles of 64 (partial ones count)
+    assert lib.pfs_get_step_tile(768, 512, 8, 0) == 32   # 96 tiles of 64, 384 of 32
+    assert lib.pfs_get_step_tile(512, 512, 8, 0) == 32
+    assert lib.pfs_get_step_tile(256, 256, 8, 0) == 16   # 64 tiles of 32
+    assert lib.pfs_get_step_tile(128, 128, 4, 1) == 16
+    assert lib.pfs_get_step_tile(1, 1, 1, 0) == 16
+    _lib.call("pfs_set_step_tile", 32)
+    try:
+        assert lib.pfs_get_step_tile(1024, 1024, 8, 1) == 32
+    finally:
+        _lib.call("pfs_set_step_tile", 0)
+    assert lib.pfs_get_step_tile(1024, 1024, 8, 1) == 64
+    for bad in (8, 48, 128, -1):
+        with pytest.raises(_lib.PfsError):
+            _lib.call("pfs_set_step_tile", bad)
+    assert lib.pfs_get_step_tile(0, 5, 8, 0) < 0
+
+
 def test_splitmix64_published_vector():
     # SplitMix64 from state 0 (Steele, Lea & Flood 2014; Vigna's splitmix64.c).
     z = scenes.splitmix64(0, 3)

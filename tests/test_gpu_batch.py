@@ -39,15 +39,40 @@ def _check(b, slot, scene, orc, steps):
     assert b.steps(slot) == steps
 
 
+@pytest.mark.parametrize("tile", [0, 16, 32, 64], ids=["auto", "t16", "t32", "t64"])
 @pytest.mark.parametrize("graphs", [True, False], ids=["graph", "launches"])
-def test_batch_mixed_scenes(orc, graphs):
+def test_batch_mixed_scenes(orc, graphs, tile):
     b = Batch(graphs=graphs)
     sc = [_scene(i, spec) for i, spec in enumerate(MIX)]
     slots = [b.add(*to_dev(s), s.dt, s.visc, s.iters) for s in sc]
     assert b.count == len(MIX)
+    b.set_tile(tile)
+    assert b.tile == (tile or 16)  # these scenes hold fewer than 128 tiles of 32
     b.step(5)
     for slot, s in zip(slots, sc):
         _check(b, slot, s, orc, 5)
+    b.close()
+
+
+def test_batch_autotune_restores_the_scenes(orc):
+    """autotune times every tile side on the live scenes, puts them back as
+    they were (buffers and step counts), and forces the fastest side."""
+    b = Batch()
+    sc = [_scene(i, spec) for i, spec in enumerate(MIX)]
+    slots = [b.add(*to_dev(s), s.dt, s.visc, s.iters) for s in sc]
+    b.step(3)
+    b.sync()
+    before = [[t.cpu().numpy().tobytes() for t in b.buffers(sl)] for sl in slots]
+    side, ms = b.autotune(reps=3)
+    assert side in (16, 32, 64) and ms[side] == min(ms.values()) and min(ms.values()) > 0
+    assert b.tile == side
+    assert [[t.cpu().numpy().tobytes() for t in b.buffers(sl)] for sl in slots] == before
+    assert all(b.steps(sl) == 3 for sl in slots)
+    b.step(2)
+    for slot, s in zip(slots, sc):
+        _check(b, slot, s, orc, 5)
+    b.set_tile(0)
+    assert b.tile == 16
     b.close()
 
 

@@ -37,6 +37,7 @@ SHAPES = [  # field w, h, dye w, h
 def _path_reset():
     yield
     fluid.set_step_path("auto")
+    fluid.set_step_tile(0)
 
 
 def _ids(shapes):
@@ -145,6 +146,40 @@ def test_steps_multi_tile(orc, shape, k):
     """Grids spanning several 64x64 tiles with partial edge tiles."""
     s = scenes.random_scene(*shape, seed=11, dt=0.5, visc=0.15, iters=k, vel_scale=40.0)
     assert_bitexact(gpu_run(s, 4), oracle_run(orc, s, 4))
+
+
+@pytest.mark.parametrize("tile", fluid.TILE_SIDES)
+@pytest.mark.parametrize("k", [1, 4, 8, 9, 17])
+@pytest.mark.parametrize("shape", [(1, 1, 1, 1), (8, 8, 8, 8), (64, 48, 96, 64), (130, 67, 130, 67),
+                                   (200, 333, 400, 111)],
+                         ids=["1x1", "8x8", "64x48-img96x64", "130x67", "200x333-img400x111"])
+def test_every_tile_side_bitexact(orc, shape, k, tile):
+    """Each tile side of the fused kernels (forced), fields from one cell to
+    several partial tiles, one and several sweep chunks, dye fused or not."""
+    fluid.set_step_tile(tile)
+    s = scenes.random_scene(*shape, seed=31 + k, dt=0.6, visc=0.2, iters=k, vel_scale=20.0)
+    assert_bitexact(gpu_run(s, 3, combined=True), oracle_run(orc, s, 3))
+
+
+def test_autotune_records_side_and_keeps_buffers(orc):
+    """autotune_step times every side on scratch copies: the buffers are
+    unchanged, the fastest side is recorded for the shape, and steps of the
+    shape then take it (still bit-exact)."""
+    fluid.clear_step_tuning()
+    s = scenes.random_scene(256, 192, 256, 192, seed=5, dt=0.5, visc=0.2, iters=6)
+    vp, tmp, im, itmp = to_dev(s)
+    before = [f.numpy().tobytes() for f in (vp, tmp, im, itmp)]
+    side, ms = fluid.autotune_step(vp, tmp, s.dt, s.visc, 6, image=im, itmp=itmp, reps=5)
+    assert side in fluid.TILE_SIDES and set(ms) == set(fluid.TILE_SIDES)
+    assert all(m > 0 for m in ms.values()) and ms[side] == min(ms.values())
+    assert [f.numpy().tobytes() for f in (vp, tmp, im, itmp)] == before
+    assert fluid.get_step_tile(256, 192, 6, dye_fused=True) == side
+    for _ in range(3):
+        fluid.fluid_timestep(vp, tmp, im, itmp, s.dt, s.visc, 6)
+    torch.cuda.synchronize()
+    assert_bitexact(tuple(f.numpy() for f in (vp, tmp, im, itmp)), oracle_run(orc, s, 3))
+    fluid.clear_step_tuning()
+    assert fluid.get_step_tile(256, 192, 6, dye_fused=True) == 16  # the rule again
 
 
 def test_steps_large_backtraces(orc):
