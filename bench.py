@@ -121,6 +121,32 @@ def cpu_baseline(scene, budget_steps: int) -> dict:
     return out
 
 
+def tiles_leg(reps: int = 50) -> dict:
+    """The fused kernels' tile side over field sizes: pfs_autotune_step times
+    a whole timestep (field and dye of one size, dt = 1/120, nu = 0.1) for
+    the sides 64, 32 and 16 on scratch copies, and the rule's side beside
+    the measured fastest.  The first shape is the 128x128, K = 4 scene."""
+    import torch
+
+    from probabilistic_fluid_simulation_amd import fluid, scenes
+
+    out = {}
+    for n, k in ((128, 4), (256, 8), (512, 8), (768, 8), (1024, 8)):
+        s = scenes.make_scene(n, n, n, n, kind="vortex", seed=0, dt=1.0 / 120.0, visc=0.1,
+                              iters=k, name=f"{n}x{n}-K{k}")
+        vp, tmp = fluid.VPField.from_numpy(s.vp), fluid.VPField.from_numpy(s.tmp)
+        im, itmp = fluid.VPField.from_numpy(s.image), fluid.VPField.from_numpy(s.itmp)
+        fluid.clear_step_tuning()
+        rule = fluid.get_step_tile(n, n, k, dye_fused=True)
+        side, ms = fluid.autotune_step(vp, tmp, s.dt, s.visc, k, image=im, itmp=itmp, reps=reps)
+        out[s.name] = {"us_per_step": {str(t): m * 1e3 for t, m in ms.items()},
+                       "fastest": side, "rule": rule}
+        del vp, tmp, im, itmp
+    fluid.clear_step_tuning()
+    torch.cuda.synchronize()
+    return out
+
+
 def batched_leg(n_scenes: int, steps: int) -> dict:
     """The batched workload: n_scenes independent scenes (dye pixels
     log-uniform in [4096, 262144], field side in {64, 128, 256, 512}, scene s
@@ -357,6 +383,7 @@ def main() -> int:
     ap.add_argument("--batched-scenes", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=300)
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-tiles", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-particles", action="store_true")
     ap.add_argument("--particle-steps", type=int, default=50)
@@ -474,6 +501,8 @@ def main() -> int:
     del vp, tmp, im, itmp
     if not args.no_batched:
         out["batched"] = batched_leg(args.batched_scenes, args.batched_steps)
+    if not args.no_tiles:
+        out["tiles"] = tiles_leg()
     if not args.no_served:
         out["served"] = [served_leg(float(r), args.served_seconds)
                          for r in args.served_rates.split(",")]
