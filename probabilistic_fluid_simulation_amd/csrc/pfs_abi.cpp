@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <climits>
 #include <cstdio>
 #include <map>
@@ -19,7 +20,7 @@ namespace pfs {
 
 namespace {
 thread_local std::string g_last_error;
-int g_step_path = PFS_PATH_AUTO;
+std::atomic<int> g_step_path{PFS_PATH_AUTO};  // process-wide settings, read by every call
 
 // One growable workspace per (device, stream): calls on one stream are
 // ordered, so they can share it; calls on different streams must not.
@@ -62,7 +63,7 @@ bool same_shape(const pfs_field *a, const pfs_field *b) {
 
 // Tile side of the fused step: the side pfs_set_step_tile forced, else the
 // side pfs_autotune_step measured fastest for this shape, else the rule.
-int g_step_tile = 0;  // 0: not forced
+std::atomic<int> g_step_tile{0};  // 0: not forced
 std::mutex g_tune_mutex;
 std::map<std::tuple<int, int, int, int>, int> g_tuned;  // (w, h, iters, dye fused) -> side
 
@@ -101,7 +102,7 @@ namespace {
 int rule_tile(int w, int h) { return tile_for_counts(tile_count(w, h, 64), tile_count(w, h, 32)); }
 
 int step_tile(int w, int h, int iters, bool dye_fused) {
-    if (g_step_tile) return g_step_tile;
+    if (const int forced = g_step_tile.load()) return forced;
     {
         std::lock_guard<std::mutex> lock(g_tune_mutex);
         auto it = g_tuned.find({w, h, iters, dye_fused ? 1 : 0});
