@@ -337,6 +337,48 @@ def particles_leg(steps: int, warmup: int = 5) -> dict:
              and np.array_equal(u32(counts), oc) and np.array_equal(u32(offsets), oo)
              and np.array_equal(u32(fused_counts), oc) and np.array_equal(u32(nbr), on)
              and rho.cpu().numpy().tobytes() == og.density(mass, by_position=True).tobytes())
+    # Density-constraint iterations (position-based fluids) on the sorted
+    # tracers: rest density = the median density, mass 1, eps 1, 4 Jacobi
+    # iterations without early exit; timed back to back on copies of the
+    # positions, then one solve replayed by the oracle.
+    pbf_iters, eps = 4, 1.0
+    rho1 = tr.density(1.0)
+    rho0 = float(rho1.median().item())
+    xs0, ys0 = tr.x.clone(), tr.y.clone()
+    px, py = xs0.clone(), ys0.clone()
+    tr.grid.pbf(1.0, rho0, eps, pbf_iters, 0.0, px, py)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        lists = tr.grid.neighbors(None, xs0, ys0, by_position=True)
+    torch.cuda.synchronize()
+    lists_ms = (time.perf_counter() - t0) / 10 * 1e3
+    lists = tuple(t.clone() for t in lists)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(10):
+        px.copy_(xs0)
+        py.copy_(ys0)
+        tr.grid.pbf(1.0, rho0, eps, pbf_iters, 0.0, px, py, lists=lists)
+    e1.record()
+    torch.cuda.synchronize()
+    pbf_ms = e0.elapsed_time(e1) / 10
+    px.copy_(xs0)
+    py.copy_(ys0)
+    lam, res, used = tr.grid.pbf(1.0, rho0, eps, pbf_iters, 0.0, px, py)
+    res = res.cpu().numpy()
+    og2 = op.Grid(xs0.cpu().numpy(), ys0.cpu().numpy(), g.domain_w, g.domain_h, r, g.gx, g.gy,
+                  g.kx, g.ky)
+    ox, oy, olam, ores, _ = og2.pbf(1.0, rho0, eps, pbf_iters)
+    pbf_exact = (px.cpu().numpy().tobytes() == ox.tobytes()
+                 and py.cpu().numpy().tobytes() == oy.tobytes()
+                 and lam.cpu().numpy().tobytes() == olam.tobytes()
+                 and bool(np.allclose(res, ores, rtol=1e-12)))
+    pbf = {"iterations": pbf_iters, "eps": eps, "rest_density": rho0,
+           "lists_ms": lists_ms, "ms_per_solve": pbf_ms, "ms_per_iteration": pbf_ms / pbf_iters,
+           "rms_C_first_last": [float(np.sqrt(res[0] / n)), float(np.sqrt(res[-1] / n))],
+           "bit_exact_vs_oracle": bool(pbf_exact),
+           "checked": "positions and lambdas bit for bit, FP64 residuals to 1e-12"}
     return {"workload": "particles-1M (1,048,576 tracers over the 1024x1024 blob image, "
                         "radius 1.5 cells)",
             "particles": n, "cells": g.gx * g.gy, "steps": steps,
@@ -346,7 +388,8 @@ def particles_leg(steps: int, warmup: int = 5) -> dict:
             "with_fluid_step_ms": both_ms,
             "bit_exact_vs_oracle": bool(exact),
             "checked": "one step replayed by the oracle: positions, ids, sort order, cell "
-                       "tables, neighbour counts, offsets, lists and densities"}
+                       "tables, neighbour counts, offsets, lists and densities",
+            "pbf": pbf}
 
 
 def served_leg(rate: float, duration: float, seed: int = 7) -> dict:

@@ -299,6 +299,29 @@ int pfs_ngrid_neighbors(const pfs_ngrid *g, const float *xs, const float *ys, co
 int pfs_ngrid_density(const pfs_ngrid *g, const float *xs, const float *ys, const uint32_t *order,
                       const uint32_t *cell_start, const uint32_t *cell_end, float mass, float *rho,
                       uint32_t *counts, void *workspace, size_t workspace_bytes, void *stream);
+/* Position-based fluids (Macklin & Mueller 2013): Jacobi iterations of the
+ * 2D density constraint C_i = rho_i / rho0 - 1 on the sorted positions
+ * (xs, ys, in place), with kernel radius h = r and the neighbour lists of
+ * those positions found at the start (pfs_ngrid_neighbors with order NULL:
+ * offsets, counts and lists by sorted position), evaluated at the current
+ * iterate:
+ *   rho_i = mass * 4/(pi h^8) * ((h^2)^3 + sum_{j in list i} (h^2 - r_ij^2)^3)
+ *   lambda_i = -C_i / (sum_k |grad_k C_i|^2 + eps)
+ *   x_i += (1/rho0) * sum_{j in list i} (lambda_i + lambda_j) grad W_spiky(x_i - x_j),
+ *   grad W_spiky(d) = -30/(pi h^5) (h - |d|)^2 d / |d|  for 0 < |d| < h, else 0,
+ * every update of an iteration from the same iterate; positions wrap.  Sums
+ * run in list order, FP32.  residuals[k] (device, max_iters + 1 doubles) =
+ * sum_i C_i^2 of iterate k in FP64 (-1 for iterates not reached); the solve
+ * stops at the first iterate with sum C^2 <= tol^2 * n (tested on the
+ * device).  lambda (device, n) gets the last iterate's lambdas; *iters_used
+ * (device, may be NULL) the number of updates applied.  ws:
+ * pfs_pbf_workspace_bytes, 256-byte aligned. */
+#define PFS_PBF_MAX_ITERS 64
+size_t pfs_pbf_workspace_bytes(const pfs_ngrid *g, int max_iters);
+int pfs_pbf_solve(const pfs_ngrid *g, float *xs, float *ys, const uint32_t *nbr_offsets,
+                  const uint32_t *nbr_counts, const uint32_t *nbr, float mass, float rho0, float eps,
+                  int max_iters, double tol, float *lambda, double *residuals, int *iters_used,
+                  void *ws, size_t ws_bytes, void *stream);
 /* out[p] = in[index[p]]: permutes per-particle data into sorted order. */
 int pfs_gather_u32(const uint32_t *index, const uint32_t *in, uint32_t *out, int n, void *stream);
 /* Exclusive prefix sum of n uint32 (wavefront DPP scans); *total (device,

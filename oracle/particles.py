@@ -25,6 +25,9 @@ def _load() -> ctypes.CDLL:
         lib.orcp_ngrid_neighbors.restype = ctypes.c_longlong
         lib.orcp_ngrid_density.argtypes = [P, P, P, I, P, P, P, I, I, F, F, F, F, P]
         lib.orcp_ngrid_density.restype = None
+        lib.orcp_pbf_solve.argtypes = [P, P, I, P, P, P, F, F, F, F, F, F, I, ctypes.c_double,
+                                       P, P, ctypes.POINTER(I)]
+        lib.orcp_pbf_solve.restype = I
         _lib = lib
     return _lib
 
@@ -84,3 +87,19 @@ class Grid:
                                    _p(self.keys_sorted), _p(self.start), _p(self.end), self.gx,
                                    self.gy, self.W, self.H, self.r, mass, _p(rho))
         return rho
+
+    def pbf(self, mass: float, rho0: float, eps: float, max_iters: int, tol: float = 0.0):
+        """The position-based-fluids solve on copies of the sorted positions,
+        with their neighbour lists: (xs, ys, lambda of the last iterate,
+        residuals [max_iters + 1], iters used)."""
+        counts, offsets, nbr = self.neighbors(by_position=True)
+        nbr = np.ascontiguousarray(nbr if len(nbr) else np.zeros(1, np.uint32))
+        xs, ys = self.xs.copy(), self.ys.copy()
+        lam = np.empty(self.n, np.float32)
+        res = np.empty(max_iters + 1, np.float64)
+        used = ctypes.c_int(0)
+        rc = _load().orcp_pbf_solve(_p(xs), _p(ys), self.n, _p(offsets), _p(counts), _p(nbr),
+                                    self.W, self.H, self.r, mass, rho0, eps, max_iters, tol,
+                                    _p(lam), _p(res), ctypes.byref(used))
+        assert rc == 0
+        return xs, ys, lam, res, used.value

@@ -152,3 +152,88 @@ void orcp_ngrid_density(const float *xs, const float *ys, const uint32_t *order,
         rho[order ? order[p] : (uint32_t)p] = mc * sum;
     }
 }
+
+/* Position-based fluids (pfs_pbf_solve; Macklin & Mueller 2013), the same
+ * float operations in the same order: per iterate k, lambda of every
+ * particle from iterate k (its own term, then its neighbour list in order),
+ * sum C^2 in FP64 (sequential here, by blocks in the library: equal to
+ * rounding), the convergence test, then every update from iterate k.  The
+ * lists (by sorted position) are those of the starting positions. */
+static float spiky_w(float h, float h2, float gc, float d2) {
+    if (!(d2 > 0.0f) || !(d2 < h2)) return 0.0f;
+    const float r = sqrtf(d2);
+    const float t = h - r;
+    return ((gc * t) * t) / r;
+}
+
+int orcp_pbf_solve(float *xs, float *ys, int n, const uint32_t *offsets, const uint32_t *counts,
+                   const uint32_t *nbr, float W, float H, float r, float mass, float rho0, float eps,
+                   int max_iters, double tol, float *lambda, double *res, int *iters_used) {
+    const float half_w = W * 0.5f, half_h = H * 0.5f, h = r, h2 = r * r;
+    const double hd = r;
+    const float coef = (float)(4.0 / (3.14159265358979323846 * hd * hd * hd * hd * hd * hd * hd * hd));
+    const float mc = mass * coef;
+    const float gc = (float)(-30.0 / (3.14159265358979323846 * hd * hd * hd * hd * hd));
+    const float r0sq = rho0 * rho0;
+    float *nx = malloc(sizeof(float) * (size_t)n), *ny = malloc(sizeof(float) * (size_t)n);
+    if (!nx || !ny) {
+        free(nx);
+        free(ny);
+        return -1;
+    }
+    for (int k = 0; k <= max_iters; ++k) res[k] = -1.0;
+    int it = 0;
+    for (;; ++it) {
+        double c2 = 0.0;
+        for (int p = 0; p < n; ++p) {
+            float S = h2 * h2 * h2, gsx = 0.0f, gsy = 0.0f, g2 = 0.0f;
+            for (uint32_t k = 0; k < counts[p]; ++k) {
+                const uint32_t q = nbr[offsets[p] + k];
+                const float dx = periodic(xs[q] - xs[p], W, half_w);
+                const float dy = periodic(ys[q] - ys[p], H, half_h);
+                const float d2 = dx * dx + dy * dy;
+                if (d2 < h2) {
+                    const float t2 = h2 - d2;
+                    S += t2 * t2 * t2;
+                }
+                const float w = spiky_w(h, h2, gc, d2);
+                if (w != 0.0f) {
+                    const float vx = w * dx, vy = w * dy;
+                    gsx += vx;
+                    gsy += vy;
+                    g2 += vx * vx + vy * vy;
+                }
+            }
+            const float rho = mc * S;
+            const float C = rho / rho0 - 1.0f;
+            const float denom = (gsx * gsx + gsy * gsy + g2) / r0sq + eps;
+            lambda[p] = -C / denom;
+            c2 += (double)C * (double)C;
+        }
+        res[it] = c2;
+        if (c2 <= tol * tol * (double)n || it == max_iters) break;
+        for (int p = 0; p < n; ++p) {
+            const float lp = lambda[p];
+            float ax = 0.0f, ay = 0.0f;
+            for (uint32_t k = 0; k < counts[p]; ++k) {
+                const uint32_t q = nbr[offsets[p] + k];
+                const float dx = periodic(xs[q] - xs[p], W, half_w);
+                const float dy = periodic(ys[q] - ys[p], H, half_h);
+                const float w = spiky_w(h, h2, gc, dx * dx + dy * dy);
+                if (w != 0.0f) {
+                    const float s = lp + lambda[q];
+                    ax += s * (w * dx);
+                    ay += s * (w * dy);
+                }
+            }
+            nx[p] = wrap_pos(xs[p] + -(ax / rho0), W);
+            ny[p] = wrap_pos(ys[p] + -(ay / rho0), H);
+        }
+        memcpy(xs, nx, sizeof(float) * (size_t)n);
+        memcpy(ys, ny, sizeof(float) * (size_t)n);
+    }
+    *iters_used = it;
+    free(nx);
+    free(ny);
+    return 0;
+}

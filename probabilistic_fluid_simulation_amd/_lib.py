@@ -21,6 +21,7 @@ PFS_ERR_UNSUPPORTED = -4
 PFS_PATH_AUTO = 0
 PFS_PATH_PER_OP = 1
 PFS_PRECISE_MAX_ITERS = 64
+PFS_PBF_MAX_ITERS = 64
 
 _ERRNAMES = {PFS_ERR_INVALID: "invalid argument", PFS_ERR_HIP: "HIP error",
              PFS_ERR_NOMEM: "out of device memory", PFS_ERR_UNSUPPORTED: "unsupported"}
@@ -112,6 +113,9 @@ SIGNATURES = {
     "pfs_particles_advect": [_V, _V, _I, _P, _F, _V],
     "pfs_ngrid_init": [_G, _I, _F, _F, _F],
     "pfs_ngrid_workspace_bytes": [_G],
+    "pfs_pbf_workspace_bytes": [_G, _I],
+    "pfs_pbf_solve": [_G, _V, _V, _V, _V, _V, _F, _F, _F, _I, ctypes.c_double, _V, _V, _V, _V,
+                      ctypes.c_size_t, _V],
     "pfs_ngrid_build": [_G, _V, _V, _V, _V, _V, _V, _V, _V, _V, ctypes.c_size_t, _V],
     "pfs_ngrid_neighbors": [_G, _V, _V, _V, _V, _V, _V, _V, _V, ctypes.c_size_t, _V, _V,
                             ctypes.c_size_t, _V],
@@ -125,7 +129,8 @@ _RESTYPES = {"pfs_version": ctypes.c_char_p, "pfs_last_error": ctypes.c_char_p,
              "pfs_step_workspace_bytes": ctypes.c_size_t,
              "pfs_batch_workspace_bytes": ctypes.c_size_t, "pfs_batch_stream": ctypes.c_void_p,
              "pfs_precise_workspace_bytes": ctypes.c_size_t,
-             "pfs_ngrid_workspace_bytes": ctypes.c_size_t}
+             "pfs_ngrid_workspace_bytes": ctypes.c_size_t,
+             "pfs_pbf_workspace_bytes": ctypes.c_size_t}
 
 _lock = threading.Lock()
 _lib = None

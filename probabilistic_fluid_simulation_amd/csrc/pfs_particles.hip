@@ -513,6 +513,179 @@ __global__ void k_tracer_advect(float *__restrict__ x, float *__restrict__ y, in
     y[i] = wrap_pos(py + div_n(dt * v, fh, inv_h, pow2_h), fh);
 }
 
+// ---- position-based fluids: the density constraint, Jacobi iterations ------
+// Macklin & Mueller, "Position Based Fluids" (ACM TOG 32(4), 2013), in 2D,
+// on the sorted positions, with the neighbour lists found at the start of
+// the solve (as the paper does once per step), kernels evaluated at the
+// current iterate:
+//   C_i = rho_i / rho0 - 1,  rho_i = m * W_poly6 summed over i and its list
+//   lambda_i = -C_i / (sum_k |grad_k C_i|^2 + eps)
+//   dx_i = (1 / rho0) * sum_{j in list} (lambda_i + lambda_j) grad W_spiky(x_i - x_j)
+// with W_poly6 = 4/(pi h^8) (h^2 - r^2)^3 and grad W_spiky(d) =
+// -30/(pi h^5) (h - r)^2 d / r for 0 < r < h (zero outside).  All updates
+// of an iteration read the same iterate (Jacobi); positions wrap.
+struct PbfCtl {
+    int done, final_it;  // converged flag; the iterate the solve ends on
+};
+
+struct PbfArgs {
+    const float *xs, *ys;  // the current iterate (sorted positions)
+    const uint32_t *noff, *ncnt, *nbr;  // neighbour lists by sorted position
+    int n;
+    float W, H, half_w, half_h;
+    float h, h2, mc, gc, rho0, r0sq, eps;
+    float *lambda;   // [n] by sorted position
+    float *nx, *ny;  // the next iterate
+    double *part;    // [max_iters + 1][nblocks]: per-block sums of C^2
+    double *res;     // [max_iters + 1]: sum of C^2 of each iterate
+    PbfCtl *ctl;
+    int it, nblocks;
+    double tol2n;    // convergence: sum C^2 <= tol^2 * n
+};
+
+// Every listed neighbour q of p, in list order: f(q, dx, dy, d2) with
+// (dx, dy) = x_q - x_p, periodic (the lists were found at the start; the
+// positions have moved since).  Loads are issued eight at a time.
+template <typename F>
+__device__ __forceinline__ void for_listed(const PbfArgs &b, int p, F &&f) {
+    const float px = b.xs[p], py = b.ys[p];
+    const uint32_t o = b.noff[p], c = b.ncnt[p];
+    auto one = [&](uint32_t q, float ox, float oy) {
+        const float dx = periodic(ox - px, b.W, b.half_w);
+        const float dy = periodic(oy - py, b.H, b.half_h);
+        f(q, dx, dy, dx * dx + dy * dy);
+    };
+    constexpr int kB = 8;  // neighbours whose loads are in flight together
+    uint32_t k = 0;
+    for (; k + kB <= c; k += kB) {
+        uint32_t q[kB];
+        float x[kB], y[kB];
+#pragma unroll
+        for (int j = 0; j < kB; ++j) q[j] = b.nbr[o + k + j];
+#pragma unroll
+        for (int j = 0; j < kB; ++j) {
+            x[j] = b.xs[q[j]];
+            y[j] = b.ys[q[j]];
+        }
+#pragma unroll
+        for (int j = 0; j < kB; ++j) one(q[j], x[j], y[j]);
+    }
+    for (; k < c; ++k) {
+        const uint32_t q = b.nbr[o + k];
+        one(q, b.xs[q], b.ys[q]);
+    }
+}
+
+// w with grad W_spiky(x_p - x_q) = -w * (dx, dy): ((gc * t) * t) / r,
+// t = h - r, r = sqrt(d2), for 0 < d2 < h^2; else 0 (selected, so that the
+// loops do not branch; adding +-0 leaves the sums unchanged).
+__device__ __forceinline__ float spiky_w(const PbfArgs &b, float d2) {
+    const float r = sqrtf(d2);
+    const float t = b.h - r;
+    const float w = ((b.gc * t) * t) / r;
+    return (d2 > 0.0f && d2 < b.h2) ? w : 0.0f;
+}
+
+// Block-wide FP64 sum (thread 0 gets it): wave shuffles, then the waves in order.
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double part[4];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += part[k];
+    return s;
+}
+
+// lambda of iterate b.it and the block sums of C^2 (256 threads).
+__global__ __launch_bounds__(256) void k_pbf_lambda(PbfArgs b) {
+    if (b.ctl->done) return;  // the same verdict for every thread
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    double c2 = 0.0;
+    if (p < b.n) {
+        float S = b.h2 * b.h2 * b.h2;  // p itself
+        float gx = 0.0f, gy = 0.0f, g2 = 0.0f;
+        for_listed(b, p, [&](uint32_t, float dx, float dy, float d2) {
+            const float t2 = fmaxf(b.h2 - d2, 0.0f);
+            S += t2 * t2 * t2;
+            const float w = spiky_w(b, d2);
+            const float vx = w * dx, vy = w * dy;
+            gx += vx;
+            gy += vy;
+            g2 += vx * vx + vy * vy;
+        });
+        const float rho = b.mc * S;
+        const float C = rho / b.rho0 - 1.0f;
+        const float denom = (gx * gx + gy * gy + g2) / b.r0sq + b.eps;
+        b.lambda[p] = -C / denom;
+        c2 = (double)C * (double)C;
+    }
+    const double s = block_sum(c2);
+    if (threadIdx.x == 0) b.part[(size_t)b.it * b.nblocks + blockIdx.x] = s;
+}
+
+// Folds iterate b.it's block sums (one block of 1024: strided sums in block
+// order, then the waves in order) and decides whether it has converged.
+__global__ __launch_bounds__(1024) void k_pbf_check(PbfArgs b) {
+    __shared__ double part[16];
+    if (b.ctl->done) return;
+    double v = 0.0;
+    for (int k = threadIdx.x; k < b.nblocks; k += 1024) v += b.part[(size_t)b.it * b.nblocks + k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int k = 0; k < 16; ++k) s += part[k];
+        b.res[b.it] = s;
+        if (s <= b.tol2n) {
+            b.ctl->done = 1;
+            b.ctl->final_it = b.it;
+        }
+    }
+}
+
+// The update of iterate b.it into the next iterate (nx, ny).
+__global__ __launch_bounds__(256) void k_pbf_delta(PbfArgs b) {
+    if (b.ctl->done) return;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= b.n) return;
+    const float lp = b.lambda[p];
+    float ax = 0.0f, ay = 0.0f;
+    for_listed(b, p, [&](uint32_t q, float dx, float dy, float d2) {
+        const float w = spiky_w(b, d2);
+        const float s = lp + b.lambda[q];
+        ax += s * (w * dx);
+        ay += s * (w * dy);
+    });
+    b.nx[p] = wrap_pos(b.xs[p] + -(ax / b.rho0), b.W);
+    b.ny[p] = wrap_pos(b.ys[p] + -(ay / b.rho0), b.H);
+}
+
+// Ends the solve on the caller's buffers: the final iterate f lives in the
+// scratch pair when f is odd.
+__global__ void k_pbf_finish(const PbfCtl *ctl, int iters, const float *sx, const float *sy,
+                             float *xs, float *ys, int *iters_used, int n) {
+    const int f = ctl->done ? ctl->final_it : iters;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0 && iters_used) *iters_used = f;
+    if (p < n && (f & 1)) {
+        xs[p] = sx[p];
+        ys[p] = sy[p];
+    }
+}
+
+__global__ void k_pbf_init(PbfCtl *ctl, double *res, int nres) {
+    if (threadIdx.x == 0) {
+        ctl->done = 0;
+        ctl->final_it = 0;
+    }
+    for (int k = threadIdx.x; k < nres; k += blockDim.x) res[k] = -1.0;  // not computed
+}
+
 int grid_for(int n) { return (n + 255) / 256; }
 
 // The digit plan for keys below ncells: the fewest passes, 8-bit digits
@@ -583,6 +756,29 @@ NArgs nargs(const pfs_ngrid *g, const float *xs, const float *ys,
     a.half_h = g->domain_h * 0.5f;
     a.r2 = g->radius * g->radius;
     return a;
+}
+
+// PBF workspace: the control block, the block sums of C^2 of every iterate,
+// and the scratch position pair.
+struct PbfWs {
+    PbfCtl *ctl;
+    double *part;
+    float *sx, *sy;
+};
+size_t pbf_part_bytes(int n, int max_iters) {
+    return ((size_t)(max_iters + 1) * grid_for(n) * sizeof(double) + 255) & ~(size_t)255;
+}
+size_t pbf_ws_bytes(int n, int max_iters) {
+    return 256 + pbf_part_bytes(n, max_iters) + 2 * (((size_t)n * sizeof(float) + 255) & ~(size_t)255);
+}
+PbfWs pbf_ws(void *ws, int n, int max_iters) {
+    char *p = static_cast<char *>(ws);
+    PbfWs w;
+    w.ctl = reinterpret_cast<PbfCtl *>(p);
+    w.part = reinterpret_cast<double *>(p + 256);
+    w.sx = reinterpret_cast<float *>(p + 256 + pbf_part_bytes(n, max_iters));
+    w.sy = w.sx + ((((size_t)n * sizeof(float) + 255) & ~(size_t)255) / sizeof(float));
+    return w;
 }
 
 }  // namespace
@@ -736,6 +932,69 @@ int pfs_ngrid_neighbors(const pfs_ngrid *g, const float *xs, const float *ys, co
     hipLaunchKernelGGL(k_nbr_fill, dim3(grid_for(g->n)), dim3(256), 0, s, a, offsets, nbr,
                        nbr_capacity);
     return check_launch("neighbour lists");
+}
+
+size_t pfs_pbf_workspace_bytes(const pfs_ngrid *g, int max_iters) {
+    if (!g || g->n < 1 || max_iters < 0 || max_iters > PFS_PBF_MAX_ITERS) return 0;
+    return pbf_ws_bytes(g->n, max_iters);
+}
+
+int pfs_pbf_solve(const pfs_ngrid *g, float *xs, float *ys, const uint32_t *nbr_offsets,
+                  const uint32_t *nbr_counts, const uint32_t *nbr, float mass, float rho0, float eps,
+                  int max_iters, double tol, float *lambda, double *residuals, int *iters_used,
+                  void *ws, size_t ws_bytes, void *stream) {
+    if (!valid_grid(g)) return set_error(PFS_ERR_INVALID, "ngrid: not initialised by pfs_ngrid_init");
+    if (!xs || !ys || !nbr_offsets || !nbr_counts || !nbr || !lambda || !residuals)
+        return set_error(PFS_ERR_INVALID, "pbf: NULL buffer");
+    if (!(rho0 > 0.0f) || !(eps > 0.0f) || !(mass > 0.0f) || !(tol >= 0.0) || max_iters < 0 ||
+        max_iters > PFS_PBF_MAX_ITERS)
+        return set_error(PFS_ERR_INVALID, "pbf: need mass, rho0, eps > 0, tol >= 0, 0 <= max_iters <= " +
+                                              std::to_string(PFS_PBF_MAX_ITERS));
+    if (!ws || ws_bytes < pbf_ws_bytes(g->n, max_iters) || ((uintptr_t)ws & 255))
+        return set_error(PFS_ERR_INVALID,
+                         "pbf: workspace not 256-byte aligned or smaller than pfs_pbf_workspace_bytes()");
+    hipStream_t s = (hipStream_t)stream;
+    const int n = g->n, nb = grid_for(n);
+    const PbfWs w = pbf_ws(ws, n, max_iters);
+    const double h = g->radius;
+    PbfArgs b{};
+    b.noff = nbr_offsets;
+    b.ncnt = nbr_counts;
+    b.nbr = nbr;
+    b.n = n;
+    b.W = g->domain_w;
+    b.H = g->domain_h;
+    b.half_w = g->domain_w * 0.5f;
+    b.half_h = g->domain_h * 0.5f;
+    b.h = g->radius;
+    b.h2 = g->radius * g->radius;
+    const float coef = (float)(4.0 / (3.14159265358979323846 * h * h * h * h * h * h * h * h));
+    b.mc = mass * coef;
+    b.gc = (float)(-30.0 / (3.14159265358979323846 * h * h * h * h * h));
+    b.rho0 = rho0;
+    b.r0sq = rho0 * rho0;
+    b.eps = eps;
+    b.lambda = lambda;
+    b.part = w.part;
+    b.res = residuals;
+    b.ctl = w.ctl;
+    b.nblocks = nb;
+    b.tol2n = tol * tol * (double)n;
+    float *bx[2] = {xs, w.sx}, *by[2] = {ys, w.sy};
+    hipLaunchKernelGGL(k_pbf_init, dim3(1), dim3(256), 0, s, w.ctl, residuals, max_iters + 1);
+    for (int it = 0; it <= max_iters; ++it) {
+        b.it = it;
+        b.xs = bx[it & 1];
+        b.ys = by[it & 1];
+        b.nx = bx[(it + 1) & 1];
+        b.ny = by[(it + 1) & 1];
+        hipLaunchKernelGGL(k_pbf_lambda, dim3(nb), dim3(256), 0, s, b);
+        hipLaunchKernelGGL(k_pbf_check, dim3(1), dim3(1024), 0, s, b);
+        if (it < max_iters) hipLaunchKernelGGL(k_pbf_delta, dim3(nb), dim3(256), 0, s, b);
+    }
+    hipLaunchKernelGGL(k_pbf_finish, dim3(nb), dim3(256), 0, s, w.ctl, max_iters, w.sx, w.sy, xs, ys,
+                       iters_used, n);
+    return check_launch("pbf solve");
 }
 
 int pfs_ngrid_density(const pfs_ngrid *g, const float *xs, const float *ys, const uint32_t *order,

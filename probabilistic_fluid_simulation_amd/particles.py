@@ -127,6 +127,39 @@ class NeighborGrid:
         return (rho, self.counts) if with_counts else rho
 
 
+    def pbf(self, mass: float, rho0: float, eps: float, max_iters: int, tol: float = 0.0,
+            xs=None, ys=None, stream=None, lists=None):
+        """Position-based-fluids density-constraint iterations (pfs_pbf_solve)
+        on the sorted positions (xs, ys: those of the last build by default),
+        in place, with their neighbour lists found first (which synchronises
+        the stream once) unless `lists` brings them.  Returns (lambda by sorted position, residuals: sum
+        C^2 per iterate in FP64, -1 where not reached, iterations used) as
+        device tensors."""
+        xs = self.xs if xs is None else xs
+        ys = self.ys if ys is None else ys
+        _check_xy(xs, ys)
+        lib = _lib.load()
+        nbytes = lib.pfs_pbf_workspace_bytes(ctypes.byref(self.g), max_iters)
+        if nbytes == 0:
+            raise ValueError(f"max_iters must be in [0, {_lib.PFS_PBF_MAX_ITERS}]")
+        # lists: (counts, offsets, nbr) of these positions by sorted position,
+        # from neighbors(by_position=True), when the caller already has them.
+        counts, offsets, nbr = (lists if lists is not None
+                                else self.neighbors(stream, xs, ys, by_position=True))
+        if getattr(self, "_pbf_ws", None) is None or self._pbf_ws.numel() < nbytes + 256:
+            self._pbf_ws = torch.empty(nbytes + 256, dtype=torch.uint8, device=xs.device)
+        base = (-self._pbf_ws.data_ptr()) % 256
+        ws = self._pbf_ws[base:base + nbytes]
+        lam = torch.empty(self.n, dtype=torch.float32, device=xs.device)
+        res = torch.empty(max_iters + 1, dtype=torch.float64, device=xs.device)
+        used = torch.zeros(1, dtype=torch.int32, device=xs.device)
+        _lib.call("pfs_pbf_solve", ctypes.byref(self.g), _ptr(xs), _ptr(ys), _ptr(offsets),
+                  _ptr(counts), _ptr(nbr), mass, rho0, eps, max_iters, tol, _ptr(lam), _ptr(res),
+                  _ptr(used), _ptr(ws), nbytes, _stream(stream))
+        self._pbf_keep = nbr  # the lists stay alive while queued work may read them
+        return lam, res, used
+
+
 def gather(index: torch.Tensor, src: torch.Tensor, out: torch.Tensor, stream=None) -> None:
     """out[p] = src[index[p]] for int32 tensors (uint32 in the C ABI)."""
     for t in (index, src, out):
@@ -172,3 +205,9 @@ class Tracers:
     def density(self, mass: float, stream=None, out=None, with_counts: bool = False):
         return self.grid.density(mass, stream, self.x, self.y, by_position=True, out=out,
                                  with_counts=with_counts)
+
+    def pbf(self, mass: float, rho0: float, eps: float, max_iters: int, tol: float = 0.0,
+            stream=None):
+        """Density-constraint iterations on the tracers' positions (in cell
+        order since the last sort), in place; see NeighborGrid.pbf."""
+        return self.grid.pbf(mass, rho0, eps, max_iters, tol, self.x, self.y, stream)

@@ -148,3 +148,65 @@ def test_tracers_kept_in_cell_order_match_oracle():
     np.testing.assert_array_equal(_u32(offsets), oo)
     np.testing.assert_array_equal(_u32(nbr), on)
     assert tr.density(0.5).cpu().numpy().tobytes() == og.density(0.5, by_position=True).tobytes()
+
+
+# ---------------------------------------------------------------- PBF solve
+def _pbf_setup(n, W, h, seed, spread):
+    rng = np.random.default_rng(seed)
+    x = (W / 2 + rng.normal(0, spread, n)).astype(np.float32) % np.float32(W)
+    y = (W / 2 + rng.normal(0, spread, n)).astype(np.float32) % np.float32(W)
+    ng = particles.NeighborGrid(n, W, W, h)
+    ng.build(torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda())
+    og = op.Grid(x, y, W, W, h, ng.g.gx, ng.g.gy, ng.g.kx, ng.g.ky)
+    rho0 = float(np.median(og.density(1.0, by_position=True)))
+    return ng, og, rho0
+
+
+@pytest.mark.parametrize("iters,stop", [(0, None), (1, None), (6, None), (8, 3), (8, 2)],
+                         ids=["0", "1", "6", "stop-at-3", "stop-at-2"])
+def test_pbf_matches_oracle(iters, stop):
+    """Positions and lambdas bit-exact to the oracle; the FP64 residuals
+    equal to rounding (block sums on the GPU); the same iterate stops it."""
+    ng, og, rho0 = _pbf_setup(5000, 40.0, 1.0, 11, 3.0)
+    eps = 50.0
+    tol = 0.0
+    if stop is not None:
+        _, _, _, res_all, _ = og.pbf(1.0, rho0, eps, iters)
+        tol = float(np.sqrt((res_all[stop - 1] + res_all[stop]) / 2 / og.n))
+    ox, oy, olam, ores, oused = og.pbf(1.0, rho0, eps, iters, tol)
+    lam, res, used = ng.pbf(1.0, rho0, eps, iters, tol)
+    torch.cuda.synchronize()
+    assert int(used.item()) == oused == (iters if stop is None else stop)
+    assert ng.xs.cpu().numpy().tobytes() == ox.tobytes()
+    assert ng.ys.cpu().numpy().tobytes() == oy.tobytes()
+    assert lam.cpu().numpy().tobytes() == olam.tobytes()
+    r = res.cpu().numpy()
+    done = ores >= 0
+    np.testing.assert_array_equal(r >= 0, done)
+    np.testing.assert_allclose(r[done], ores[done], rtol=1e-12)
+
+
+def test_pbf_on_tracers_and_errors():
+    """Tracers.pbf works on the tracers' own (cell-ordered) positions; bad
+    arguments are rejected before anything runs."""
+    s, x, y, r = scenes.particles_16k()
+    H, W = s.vp.shape[:2]
+    tr = particles.Tracers(torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda(), float(W),
+                           float(H), r)
+    tr.sort()
+    rho = tr.density(1.0)
+    rho0 = float(rho.median().item())
+    og = op.Grid(tr.x.cpu().numpy(), tr.y.cpu().numpy(), float(W), float(H), r, tr.grid.g.gx,
+                 tr.grid.g.gy, tr.grid.g.kx, tr.grid.g.ky)
+    ox, oy, olam, ores, oused = og.pbf(1.0, rho0, 20.0, 4)
+    lam, res, used = tr.pbf(1.0, rho0, 20.0, 4)
+    torch.cuda.synchronize()
+    assert tr.x.cpu().numpy().tobytes() == ox.tobytes()
+    assert tr.y.cpu().numpy().tobytes() == oy.tobytes()
+    assert np.all(np.diff(res.cpu().numpy()) < 0)
+    from probabilistic_fluid_simulation_amd import _lib
+    for bad in (dict(max_iters=65), dict(max_iters=-1)):
+        with pytest.raises(ValueError):
+            tr.pbf(1.0, rho0, 20.0, **bad)
+    with pytest.raises(_lib.PfsError):
+        tr.pbf(1.0, -1.0, 20.0, 2)

@@ -166,3 +166,96 @@ def test_tracers_stay_in_domain_and_follow_a_uniform_field():
     dx = np.abs((x - ex + w / 2) % w - w / 2)
     dy = np.abs((y - ey + h / 2) % h - h / 2)
     assert dx.max() < 1e-4 and dy.max() < 1e-4
+
+
+# ---------------------------------------------------------------- PBF solve
+def _blob(n, W, seed, spread):
+    rng = np.random.default_rng(seed)
+    x = (W / 2 + rng.normal(0, spread, n)).astype(np.float32) % np.float32(W)
+    y = (W / 2 + rng.normal(0, spread, n)).astype(np.float32) % np.float32(W)
+    return x, y
+
+
+def _pbf64(xs, ys, lam_in, W, h, mass, rho0, eps):
+    """FP64 restatement over all pairs within h (periodic), from float32
+    positions: (lambda, displacement) of one iteration; lam_in: the lambdas
+    the displacement uses (the solve's own, so that it is pinned separately)."""
+    x = xs.astype(np.float64)
+    y = ys.astype(np.float64)
+    dx = x[None, :] - x[:, None]        # x_q - x_p
+    dy = y[None, :] - y[:, None]
+    dx = np.where(dx > W / 2, dx - W, np.where(dx < -W / 2, dx + W, dx))
+    dy = np.where(dy > W / 2, dy - W, np.where(dy < -W / 2, dy + W, dy))
+    d2 = dx * dx + dy * dy
+    inside = d2 < h * h
+    rho = mass * 4 / (np.pi * h ** 8) * np.where(inside, (h * h - d2) ** 3, 0).sum(1)
+    C = rho / rho0 - 1
+    r = np.sqrt(d2)
+    pair = inside & (d2 > 0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        w = np.where(pair, -30 / (np.pi * h ** 5) * (h - r) ** 2 / r, 0)  # grad W(p - q) = -w d_qp
+    vx, vy = w * dx, w * dy
+    g2 = vx.sum(1) ** 2 + vy.sum(1) ** 2 + (vx * vx + vy * vy).sum(1)
+    lam = -C / (g2 / rho0 ** 2 + eps)
+    s = lam_in[:, None].astype(np.float64) + lam_in[None, :].astype(np.float64)
+    disp_x = -(s * vx).sum(1) / rho0
+    disp_y = -(s * vy).sum(1) / rho0
+    return lam, disp_x, disp_y, C
+
+
+def _pbf_grid(n=1500, W=24.0, h=1.0, seed=3, spread=2.0):
+    x, y = _blob(n, W, seed, spread)
+    g, og = _oracle(x, y, W, W, h)
+    rho = og.density(1.0, by_position=True)
+    rho0 = float(np.median(rho))
+    return og, rho0
+
+
+def test_pbf_lambda_and_update_against_fp64():
+    """lambda of iterate 0 and the first update against an FP64 restatement
+    over all pairs; the update is pinned with the solve's own lambdas."""
+    og, rho0 = _pbf_grid()
+    W, h, eps = og.W, og.r, 50.0
+    _, _, lam0, res0, used0 = og.pbf(1.0, rho0, eps, 0)  # no update: iterate 0's lambdas
+    assert used0 == 0
+    lam64, dx64, dy64, C = _pbf64(og.xs, og.ys, lam0, W, h, 1.0, rho0, eps)
+    lam_tol = 2e-5 * np.abs(lam64).max()
+    np.testing.assert_allclose(lam0, lam64, rtol=0, atol=lam_tol)
+    assert res0[0] == pytest.approx((C ** 2).sum(), rel=1e-5)
+    xs1, ys1, _, _, used1 = og.pbf(1.0, rho0, eps, 1)
+    assert used1 == 1
+    ddx = (xs1.astype(np.float64) - og.xs + W / 2) % W - W / 2
+    ddy = (ys1.astype(np.float64) - og.ys + W / 2) % W - W / 2
+    d_tol = 1e-4 * max(np.abs(dx64).max(), np.abs(dy64).max()) + 4e-6 * W
+    np.testing.assert_allclose(ddx, dx64, rtol=0, atol=d_tol)
+    np.testing.assert_allclose(ddy, dy64, rtol=0, atol=d_tol)
+    # a flipped sign of the update (or of lambda) would miss by 2|d|, far outside
+    assert 2 * np.abs(dx64).max() > 50 * d_tol
+    assert 2 * np.abs(lam64).max() > 50 * lam_tol
+
+
+def test_pbf_update_conserves_momentum():
+    """Pair terms (lambda_i + lambda_j) grad W(x_i - x_j) are antisymmetric,
+    so the displacements of one iteration sum to zero (up to rounding)."""
+    og, rho0 = _pbf_grid(seed=4)
+    xs1, ys1, lam, res, used = og.pbf(1.0, rho0, 50.0, 1)
+    ddx = (xs1.astype(np.float64) - og.xs + og.W / 2) % og.W - og.W / 2
+    ddy = (ys1.astype(np.float64) - og.ys + og.W / 2) % og.W - og.W / 2
+    assert abs(ddx.sum()) < 1e-3 * np.abs(ddx).sum()
+    assert abs(ddy.sum()) < 1e-3 * np.abs(ddy).sum()
+
+
+def test_pbf_residual_falls_and_early_exit():
+    og, rho0 = _pbf_grid(seed=5)
+    n = og.n
+    _, _, _, res, used = og.pbf(1.0, rho0, 50.0, 8)
+    assert used == 8 and np.all(np.diff(res) < 0), res
+    # stop at the first iterate with sum C^2 <= tol^2 n: iterate 3
+    tol = float(np.sqrt((res[2] + res[3]) / 2 / n))
+    xs3, ys3, _, res3, used3 = og.pbf(1.0, rho0, 50.0, 8, tol)
+    assert used3 == 3
+    np.testing.assert_array_equal(res3[:4], res[:4])
+    assert np.all(res3[4:] == -1.0)
+    xs_ref, ys_ref, _, _, _ = og.pbf(1.0, rho0, 50.0, 3)
+    np.testing.assert_array_equal(xs3, xs_ref)
+    np.testing.assert_array_equal(ys3, ys_ref)
