@@ -181,6 +181,10 @@ def batched_leg(n_scenes: int, steps: int) -> dict:
     b = Batch()
     batched = [clone(x) for x in inputs]
     slots = [b.add(*bufs, j.dt, j.visc, j.iters) for j, bufs in zip(jobs, batched)]
+    # The tile side: the rule's, and the autotuner's measurement on these scenes
+    # (it restores them, then forces the fastest side for the timed run).
+    rule_side = b.tile
+    tuned_side, tuned_ms = b.autotune(reps=3)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     b.step(steps)
@@ -194,6 +198,8 @@ def batched_leg(n_scenes: int, steps: int) -> dict:
             "jobs_per_s": n_scenes / t_batch, "plain_loop_jobs_per_s": n_scenes / t_plain,
             "speedup_vs_plain_loop": t_plain / t_batch,
             "cell_steps_per_s": cells * steps / t_batch, "results_identical": bool(same),
+            "tile": {"rule": rule_side, "autotuned": tuned_side,
+                     "ms_per_step": {str(k): v for k, v in tuned_ms.items()}},
             "entry_point": "pfs_batch_step (hipGraph replay)"}
 
 

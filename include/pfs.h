@@ -301,8 +301,9 @@ int pfs_ngrid_density(const pfs_ngrid *g, const float *xs, const float *ys, cons
                       uint32_t *counts, void *workspace, size_t workspace_bytes, void *stream);
 /* Position-based fluids (Macklin & Mueller 2013): Jacobi iterations of the
  * 2D density constraint C_i = rho_i / rho0 - 1 on the sorted positions
- * (xs, ys, in place), with kernel radius h = r and the neighbour lists of
- * those positions found at the start (pfs_ngrid_neighbors with order NULL:
+ * (xs, ys, in place), with kernel radius h = r, the cell tables of the last
+ * pfs_ngrid_build (whose workspace ngrid_ws holds the sorted keys) and the
+ * neighbour lists of those positions (pfs_ngrid_neighbors with order NULL:
  * offsets, counts and lists by sorted position), evaluated at the current
  * iterate:
  *   rho_i = mass * 4/(pi h^8) * ((h^2)^3 + sum_{j in list i} (h^2 - r_ij^2)^3)
@@ -318,10 +319,11 @@ int pfs_ngrid_density(const pfs_ngrid *g, const float *xs, const float *ys, cons
  * pfs_pbf_workspace_bytes, 256-byte aligned. */
 #define PFS_PBF_MAX_ITERS 64
 size_t pfs_pbf_workspace_bytes(const pfs_ngrid *g, int max_iters);
-int pfs_pbf_solve(const pfs_ngrid *g, float *xs, float *ys, const uint32_t *nbr_offsets,
-                  const uint32_t *nbr_counts, const uint32_t *nbr, float mass, float rho0, float eps,
-                  int max_iters, double tol, float *lambda, double *residuals, int *iters_used,
-                  void *ws, size_t ws_bytes, void *stream);
+int pfs_pbf_solve(const pfs_ngrid *g, float *xs, float *ys, const uint32_t *cell_start,
+                  const uint32_t *cell_end, const uint32_t *nbr_offsets, const uint32_t *nbr_counts,
+                  const uint32_t *nbr, float mass, float rho0, float eps, int max_iters, double tol,
+                  float *lambda, double *residuals, int *iters_used, void *ngrid_ws,
+                  size_t ngrid_ws_bytes, void *ws, size_t ws_bytes, void *stream);
 /* out[p] = in[index[p]]: permutes per-particle data into sorted order. */
 int pfs_gather_u32(const uint32_t *index, const uint32_t *in, uint32_t *out, int n, void *stream);
 /* Exclusive prefix sum of n uint32 (wavefront DPP scans); *total (device,

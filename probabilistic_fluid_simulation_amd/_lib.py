@@ -114,8 +114,8 @@ SIGNATURES = {
     "pfs_ngrid_init": [_G, _I, _F, _F, _F],
     "pfs_ngrid_workspace_bytes": [_G],
     "pfs_pbf_workspace_bytes": [_G, _I],
-    "pfs_pbf_solve": [_G, _V, _V, _V, _V, _V, _F, _F, _F, _I, ctypes.c_double, _V, _V, _V, _V,
-                      ctypes.c_size_t, _V],
+    "pfs_pbf_solve": [_G, _V, _V, _V, _V, _V, _V, _V, _F, _F, _F, _I, ctypes.c_double, _V, _V,
+                      _V, _V, ctypes.c_size_t, _V, ctypes.c_size_t, _V],
     "pfs_ngrid_build": [_G, _V, _V, _V, _V, _V, _V, _V, _V, _V, ctypes.c_size_t, _V],
     "pfs_ngrid_neighbors": [_G, _V, _V, _V, _V, _V, _V, _V, _V, ctypes.c_size_t, _V, _V,
                             ctypes.c_size_t, _V],

@@ -531,6 +531,8 @@ struct PbfCtl {
 struct PbfArgs {
     const float *xs, *ys;  // the current iterate (sorted positions)
     const uint32_t *noff, *ncnt, *nbr;  // neighbour lists by sorted position
+    const uint32_t *sk, *cstart, *cend;  // the start's sorted keys and cell tables
+    int gx, gy;
     int n;
     float W, H, half_w, half_h;
     float h, h2, mc, gc, rho0, r0sq, eps;
@@ -543,36 +545,111 @@ struct PbfArgs {
     double tol2n;    // convergence: sum C^2 <= tol^2 * n
 };
 
-// Every listed neighbour q of p, in list order: f(q, dx, dy, d2) with
+// The positions a workgroup's 256 particles gather, staged in LDS.  Their
+// lists lie in the three row windows of their cells' 3x3 blocks: when the
+// particles share one cell row away from the domain's edges, row cy + oy
+// contributes the sorted positions [start(cy + oy, cx_first - 1),
+// end(cy + oy, cx_last + 1)), a few hundred in all.  Staged is uniform over
+// the workgroup; otherwise (several rows, an edge, or more than kStage
+// positions) the gathers go to memory.
+constexpr int kStage = 2048;
+struct Staged {
+    float x[kStage], y[kStage], l[kStage];
+    uint32_t w0, w1, w2, len0, len01, len012;  // window starts; cumulative lengths
+    int on;
+};
+
+template <bool WITH_LAMBDA>
+__device__ __forceinline__ void stage_windows(const PbfArgs &b, Staged &st) {
+    if (threadIdx.x == 0) {
+        const int p0 = blockIdx.x * blockDim.x;
+        const int pl = min(p0 + (int)blockDim.x, b.n) - 1;
+        const uint32_t c0 = b.sk[p0], c1 = b.sk[pl];
+        const int cy = (int)(c0 / (uint32_t)b.gx), cx0 = (int)(c0 % (uint32_t)b.gx),
+                  cx1 = (int)(c1 % (uint32_t)b.gx);
+        int on = (int)(c1 / (uint32_t)b.gx) == cy && cy >= 1 && cy <= b.gy - 2 && cx0 >= 1 &&
+                 cx1 <= b.gx - 2 && c0 < (uint32_t)b.gx * (uint32_t)b.gy &&
+                 c1 < (uint32_t)b.gx * (uint32_t)b.gy;
+        uint32_t ws[3] = {0, 0, 0}, len[3] = {0, 0, 0};
+        if (on) {
+            for (int k = 0; k < 3; ++k) {
+                const uint32_t row = (uint32_t)((cy - 1 + k) * b.gx);
+                ws[k] = b.cstart[row + cx0 - 1];
+                const uint32_t we = b.cend[row + cx1 + 1];
+                len[k] = we > ws[k] ? we - ws[k] : 0u;
+            }
+            on = len[0] + len[1] + len[2] <= (uint32_t)kStage;
+        }
+        st.w0 = ws[0];
+        st.w1 = ws[1];
+        st.w2 = ws[2];
+        st.len0 = len[0];
+        st.len01 = len[0] + len[1];
+        st.len012 = len[0] + len[1] + len[2];
+        st.on = on;
+    }
+    __syncthreads();
+    if (!st.on) return;
+    // Window k's positions go to slots [len_0 + .. + len_(k-1), + len_k).
+    for (uint32_t j = threadIdx.x; j < st.len012; j += blockDim.x) {
+        uint32_t q;
+        if (j < st.len0) q = st.w0 + j;
+        else if (j < st.len01) q = st.w1 + (j - st.len0);
+        else q = st.w2 + (j - st.len01);
+        if (q < (uint32_t)b.n) {
+            st.x[j] = b.xs[q];
+            st.y[j] = b.ys[q];
+            if (WITH_LAMBDA) st.l[j] = b.lambda[q];
+        }
+    }
+    __syncthreads();
+}
+
+// The LDS slot of sorted position q, or kStage when q is in no window (the
+// lists then did not come from these cells; such entries read memory).
+__device__ __forceinline__ uint32_t staged_slot(const Staged &st, uint32_t q) {
+    if (q >= st.w0 && q - st.w0 < st.len0) return q - st.w0;
+    if (q >= st.w1 && q - st.w1 < st.len01 - st.len0) return q - st.w1 + st.len0;
+    if (q >= st.w2 && q - st.w2 < st.len012 - st.len01) return q - st.w2 + st.len01;
+    return (uint32_t)kStage;
+}
+
+// Every listed neighbour q of p, in list order: f(q, dx, dy, d2, slot) with
 // (dx, dy) = x_q - x_p, periodic (the lists were found at the start; the
-// positions have moved since).  Loads are issued eight at a time.
+// positions have moved since) and slot q's LDS slot (kStage: not staged).
+// Loads are issued eight at a time.
 template <typename F>
-__device__ __forceinline__ void for_listed(const PbfArgs &b, int p, F &&f) {
+__device__ __forceinline__ void for_listed(const PbfArgs &b, const Staged &st, int p, F &&f) {
     const float px = b.xs[p], py = b.ys[p];
     const uint32_t o = b.noff[p], c = b.ncnt[p];
-    auto one = [&](uint32_t q, float ox, float oy) {
+    const bool on = st.on;
+    auto one = [&](uint32_t q, uint32_t slot, float ox, float oy) {
         const float dx = periodic(ox - px, b.W, b.half_w);
         const float dy = periodic(oy - py, b.H, b.half_h);
-        f(q, dx, dy, dx * dx + dy * dy);
+        f(q, slot, dx, dy, dx * dx + dy * dy);
     };
     constexpr int kB = 8;  // neighbours whose loads are in flight together
     uint32_t k = 0;
     for (; k + kB <= c; k += kB) {
-        uint32_t q[kB];
+        uint32_t q[kB], sl[kB];
         float x[kB], y[kB];
 #pragma unroll
         for (int j = 0; j < kB; ++j) q[j] = b.nbr[o + k + j];
 #pragma unroll
         for (int j = 0; j < kB; ++j) {
-            x[j] = b.xs[q[j]];
-            y[j] = b.ys[q[j]];
+            sl[j] = on ? staged_slot(st, q[j]) : (uint32_t)kStage;
+            const bool lds = sl[j] < (uint32_t)kStage;
+            x[j] = lds ? st.x[lds ? sl[j] : 0] : b.xs[q[j]];
+            y[j] = lds ? st.y[lds ? sl[j] : 0] : b.ys[q[j]];
         }
 #pragma unroll
-        for (int j = 0; j < kB; ++j) one(q[j], x[j], y[j]);
+        for (int j = 0; j < kB; ++j) one(q[j], sl[j], x[j], y[j]);
     }
     for (; k < c; ++k) {
         const uint32_t q = b.nbr[o + k];
-        one(q, b.xs[q], b.ys[q]);
+        const uint32_t sl = on ? staged_slot(st, q) : (uint32_t)kStage;
+        const bool lds = sl < (uint32_t)kStage;
+        one(q, sl, lds ? st.x[lds ? sl : 0] : b.xs[q], lds ? st.y[lds ? sl : 0] : b.ys[q]);
     }
 }
 
@@ -602,12 +679,14 @@ __device__ __forceinline__ double block_sum(double v) {
 // lambda of iterate b.it and the block sums of C^2 (256 threads).
 __global__ __launch_bounds__(256) void k_pbf_lambda(PbfArgs b) {
     if (b.ctl->done) return;  // the same verdict for every thread
+    __shared__ Staged st;
+    stage_windows<false>(b, st);
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     double c2 = 0.0;
     if (p < b.n) {
         float S = b.h2 * b.h2 * b.h2;  // p itself
         float gx = 0.0f, gy = 0.0f, g2 = 0.0f;
-        for_listed(b, p, [&](uint32_t, float dx, float dy, float d2) {
+        for_listed(b, st, p, [&](uint32_t, uint32_t, float dx, float dy, float d2) {
             const float t2 = fmaxf(b.h2 - d2, 0.0f);
             S += t2 * t2 * t2;
             const float w = spiky_w(b, d2);
@@ -651,13 +730,16 @@ __global__ __launch_bounds__(1024) void k_pbf_check(PbfArgs b) {
 // The update of iterate b.it into the next iterate (nx, ny).
 __global__ __launch_bounds__(256) void k_pbf_delta(PbfArgs b) {
     if (b.ctl->done) return;
+    __shared__ Staged st;
+    stage_windows<true>(b, st);
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= b.n) return;
     const float lp = b.lambda[p];
     float ax = 0.0f, ay = 0.0f;
-    for_listed(b, p, [&](uint32_t q, float dx, float dy, float d2) {
+    for_listed(b, st, p, [&](uint32_t q, uint32_t slot, float dx, float dy, float d2) {
         const float w = spiky_w(b, d2);
-        const float s = lp + b.lambda[q];
+        const bool lds = slot < (uint32_t)kStage;
+        const float s = lp + (lds ? st.l[lds ? slot : 0] : b.lambda[q]);
         ax += s * (w * dx);
         ay += s * (w * dy);
     });
@@ -939,13 +1021,17 @@ size_t pfs_pbf_workspace_bytes(const pfs_ngrid *g, int max_iters) {
     return pbf_ws_bytes(g->n, max_iters);
 }
 
-int pfs_pbf_solve(const pfs_ngrid *g, float *xs, float *ys, const uint32_t *nbr_offsets,
-                  const uint32_t *nbr_counts, const uint32_t *nbr, float mass, float rho0, float eps,
-                  int max_iters, double tol, float *lambda, double *residuals, int *iters_used,
-                  void *ws, size_t ws_bytes, void *stream) {
+int pfs_pbf_solve(const pfs_ngrid *g, float *xs, float *ys, const uint32_t *cell_start,
+                  const uint32_t *cell_end, const uint32_t *nbr_offsets, const uint32_t *nbr_counts,
+                  const uint32_t *nbr, float mass, float rho0, float eps, int max_iters, double tol,
+                  float *lambda, double *residuals, int *iters_used, void *ngrid_ws,
+                  size_t ngrid_ws_bytes, void *ws, size_t ws_bytes, void *stream) {
     if (!valid_grid(g)) return set_error(PFS_ERR_INVALID, "ngrid: not initialised by pfs_ngrid_init");
-    if (!xs || !ys || !nbr_offsets || !nbr_counts || !nbr || !lambda || !residuals)
+    if (!xs || !ys || !cell_start || !cell_end || !nbr_offsets || !nbr_counts || !nbr || !lambda ||
+        !residuals)
         return set_error(PFS_ERR_INVALID, "pbf: NULL buffer");
+    if (!ngrid_ws || ngrid_ws_bytes < pfs_ngrid_workspace_bytes(g) || ((uintptr_t)ngrid_ws & 3))
+        return set_error(PFS_ERR_INVALID, "pbf: neighbour-grid workspace too small or misaligned");
     if (!(rho0 > 0.0f) || !(eps > 0.0f) || !(mass > 0.0f) || !(tol >= 0.0) || max_iters < 0 ||
         max_iters > PFS_PBF_MAX_ITERS)
         return set_error(PFS_ERR_INVALID, "pbf: need mass, rho0, eps > 0, tol >= 0, 0 <= max_iters <= " +
@@ -961,6 +1047,11 @@ int pfs_pbf_solve(const pfs_ngrid *g, float *xs, float *ys, const uint32_t *nbr_
     b.noff = nbr_offsets;
     b.ncnt = nbr_counts;
     b.nbr = nbr;
+    b.sk = sort_ws(ngrid_ws, n).k0;
+    b.cstart = cell_start;
+    b.cend = cell_end;
+    b.gx = g->gx;
+    b.gy = g->gy;
     b.n = n;
     b.W = g->domain_w;
     b.H = g->domain_h;

@@ -153,9 +153,10 @@ class NeighborGrid:
         lam = torch.empty(self.n, dtype=torch.float32, device=xs.device)
         res = torch.empty(max_iters + 1, dtype=torch.float64, device=xs.device)
         used = torch.zeros(1, dtype=torch.int32, device=xs.device)
-        _lib.call("pfs_pbf_solve", ctypes.byref(self.g), _ptr(xs), _ptr(ys), _ptr(offsets),
-                  _ptr(counts), _ptr(nbr), mass, rho0, eps, max_iters, tol, _ptr(lam), _ptr(res),
-                  _ptr(used), _ptr(ws), nbytes, _stream(stream))
+        _lib.call("pfs_pbf_solve", ctypes.byref(self.g), _ptr(xs), _ptr(ys), _ptr(self.cell_start),
+                  _ptr(self.cell_end), _ptr(offsets), _ptr(counts), _ptr(nbr), mass, rho0, eps,
+                  max_iters, tol, _ptr(lam), _ptr(res), _ptr(used), _ptr(self.ws),
+                  self.ws.numel() * 4, _ptr(ws), nbytes, _stream(stream))
         self._pbf_keep = nbr  # the lists stay alive while queued work may read them
         return lam, res, used
 
