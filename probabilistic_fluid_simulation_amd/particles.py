@@ -146,6 +146,8 @@ class NeighborGrid:
         # from neighbors(by_position=True), when the caller already has them.
         counts, offsets, nbr = (lists if lists is not None
                                 else self.neighbors(stream, xs, ys, by_position=True))
+        if nbr.numel() == 0:  # no neighbours at all: the C ABI still wants a buffer
+            nbr = torch.zeros(1, dtype=torch.int32, device=xs.device)
         if getattr(self, "_pbf_ws", None) is None or self._pbf_ws.numel() < nbytes + 256:
             self._pbf_ws = torch.empty(nbytes + 256, dtype=torch.uint8, device=xs.device)
         base = (-self._pbf_ws.data_ptr()) % 256

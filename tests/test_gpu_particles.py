@@ -210,3 +210,18 @@ def test_pbf_on_tracers_and_errors():
             tr.pbf(1.0, rho0, 20.0, **bad)
     with pytest.raises(_lib.PfsError):
         tr.pbf(1.0, -1.0, 20.0, 2)
+
+
+@pytest.mark.parametrize("n", [1, 3, 300])
+def test_pbf_small_counts(n):
+    """One particle (no neighbours), a few, and fewer than one workgroup."""
+    ng, og, rho0 = _pbf_setup(n, 12.0, 1.0, 17, 0.8)
+    ox, oy, olam, ores, oused = og.pbf(1.0, rho0, 10.0, 3)
+    lam, res, used = ng.pbf(1.0, rho0, 10.0, 3)
+    torch.cuda.synchronize()
+    # one particle is its own rest density: sum C^2 = 0 <= tol^2 n stops at iterate 0
+    assert int(used.item()) == oused == (0 if n == 1 else 3)
+    assert ng.xs.cpu().numpy().tobytes() == ox.tobytes()
+    assert ng.ys.cpu().numpy().tobytes() == oy.tobytes()
+    assert lam.cpu().numpy().tobytes() == olam.tobytes()
+    np.testing.assert_allclose(res.cpu().numpy(), ores, rtol=1e-12)
