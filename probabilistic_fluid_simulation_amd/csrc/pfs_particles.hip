@@ -457,6 +457,48 @@ __global__ void k_nbr_fill(NArgs a, const uint32_t *__restrict__ offsets, uint32
     });
 }
 
+// The same lists by sorted position (order null), written through LDS: the
+// lists of a wave's 64 consecutive particles are one contiguous block of the
+// output, so each wave gathers its entries in an LDS window of kFillWin
+// slots and then writes the window out with consecutive lanes (a lane's
+// own list would otherwise go out as one scattered word per store).  A
+// block larger than the window takes further traversals, one per window.
+// Waves work independently: the LDS handoff is within one wave, whose LDS
+// accesses complete in order.
+constexpr int kFillWin = 2048;
+__global__ __launch_bounds__(256) void k_nbr_fill_lds(NArgs a, const uint32_t *__restrict__ offsets,
+                                                      const uint32_t *__restrict__ counts,
+                                                      uint32_t *__restrict__ nbr, size_t capacity) {
+    __shared__ uint32_t win[4][kFillWin];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int pw = blockIdx.x * blockDim.x + wave * 64;  // the wave's first particle
+    if (pw >= a.n) return;                              // the whole wave is past the end
+    const int plast = min(pw + 63, a.n - 1);
+    const size_t base = offsets[pw], end = (size_t)offsets[plast] + counts[plast];
+    uint32_t *buf = win[wave];
+    for (size_t w0 = base; w0 < end; w0 += kFillWin) {
+        if (p < a.n) {
+            size_t o = offsets[p];
+            for_neighbors(a, p, [&](uint32_t q, float d2) {
+                if (d2 < a.r2 && (int)q != p) {
+                    if (o >= w0 && o < w0 + kFillWin) buf[o - w0] = q;
+                    ++o;
+                }
+            });
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const size_t stop = end - w0 < (size_t)kFillWin ? end - w0 : (size_t)kFillWin;
+        for (size_t j = lane; j < stop; j += 64)
+            if (w0 + j < capacity) nbr[w0 + j] = buf[j];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
 // SPH density with the 2D poly6 kernel, W(d) = c (r^2 - d^2)^3; with counts,
 // the neighbour counts of the same traversal.
 // rho = mc * S with mc = mass * 4/(pi r^8) and S the FP32 sum, in traversal
@@ -1011,8 +1053,12 @@ int pfs_ngrid_neighbors(const pfs_ngrid *g, const float *xs, const float *ys, co
         return set_error(PFS_ERR_INVALID, "ngrid: neighbour list capacity " +
                                               std::to_string(nbr_capacity) + " < " +
                                               std::to_string(h_total));
-    hipLaunchKernelGGL(k_nbr_fill, dim3(grid_for(g->n)), dim3(256), 0, s, a, offsets, nbr,
-                       nbr_capacity);
+    if (order)
+        hipLaunchKernelGGL(k_nbr_fill, dim3(grid_for(g->n)), dim3(256), 0, s, a, offsets, nbr,
+                           nbr_capacity);
+    else
+        hipLaunchKernelGGL(k_nbr_fill_lds, dim3(grid_for(g->n)), dim3(256), 0, s, a, offsets,
+                           counts, nbr, nbr_capacity);
     return check_launch("neighbour lists");
 }
 

@@ -225,3 +225,24 @@ def test_pbf_small_counts(n):
     assert ng.ys.cpu().numpy().tobytes() == oy.tobytes()
     assert lam.cpu().numpy().tobytes() == olam.tobytes()
     np.testing.assert_allclose(res.cpu().numpy(), ores, rtol=1e-12)
+
+
+@pytest.mark.parametrize("spread", [2.5, 4.0], ids=["dense", "sparse"])
+def test_lists_by_position_bitexact(spread):
+    """Lists by sorted position (written through LDS windows): a dense blob
+    puts more entries in a wave's block than one window holds."""
+    n, W = 20000, 40.0
+    rng = np.random.default_rng(23)
+    x = (W / 2 + rng.normal(0, spread * 4, n)).astype(np.float32) % np.float32(W)
+    y = (W / 2 + rng.normal(0, spread * 4, n)).astype(np.float32) % np.float32(W)
+    ng = particles.NeighborGrid(n, W, W, 1.0)
+    ng.build(torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda())
+    og = op.Grid(x, y, W, W, 1.0, ng.g.gx, ng.g.gy, ng.g.kx, ng.g.ky)
+    counts, offsets, nbr = ng.neighbors(by_position=True)
+    oc, oo, on = og.neighbors(by_position=True)
+    np.testing.assert_array_equal(_u32(counts), oc)
+    np.testing.assert_array_equal(_u32(offsets), oo)
+    np.testing.assert_array_equal(_u32(nbr), on)
+    if spread < 3:
+        per_wave = np.add.reduceat(oc.astype(np.int64), np.arange(0, n, 64))
+        assert per_wave.max() > 2048  # several windows for some waves
