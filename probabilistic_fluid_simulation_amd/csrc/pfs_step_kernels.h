@@ -8,7 +8,7 @@
 
 namespace pfs {
 
-constexpr int kTile = 64;    // the largest tile side, and the batched form's
+constexpr int kTile = 64;    // the largest tile side (the default)
 constexpr int kSegs = 12;    // row segments per staged column of a 64-wide tile
 constexpr int kMaxChunk = 8; // sweeps per launch
 
