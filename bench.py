@@ -445,8 +445,8 @@ def main() -> int:
                          "kernel of known traffic for calibrating HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(how the step would shard is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: the library runs on one GPU; --gpus {args.gpus} is not supported",
+              file=sys.stderr)
         return 2
 
     import torch
