@@ -39,7 +39,7 @@ extern "C" {
 #define PFS_ERR_INVALID (-1)     /* bad argument: null pointer, size, channel count, aliasing */
 #define PFS_ERR_HIP (-2)         /* a HIP runtime call failed */
 #define PFS_ERR_NOMEM (-3)       /* device allocation failed */
-#define PFS_ERR_UNSUPPORTED (-4) /* valid request outside what this build implements */
+#define PFS_ERR_UNSUPPORTED (-4) /* valid request outside what the library implements */
 
 /* Same member layout as the reference's vp_field (includes/fluid.hpp:17-22). */
 typedef struct pfs_field {
