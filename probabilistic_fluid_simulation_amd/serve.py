@@ -48,8 +48,10 @@ class Job:
     @property
     def nbytes(self) -> int:
         """Device bytes while resident: vp, tmp (16 B/cell each), image and
-        itmp (16 B/pixel each) and the step workspace (8 B/cell for K <= 8)."""
-        return 32 * self.fw * self.fh + 32 * self.iw * self.ih + 8 * self.fw * self.fh
+        itmp (16 B/pixel each) and the batch's scene workspace (16 B/cell for
+        K <= 8, 32 B/cell beyond; pfs_batch_workspace_bytes)."""
+        ws = 16 if self.iters <= 8 else 32
+        return (32 + ws) * self.fw * self.fh + 32 * self.iw * self.ih
 
     @property
     def latency(self) -> Optional[float]:
