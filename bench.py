@@ -121,6 +121,54 @@ def cpu_baseline(scene, budget_steps: int) -> dict:
     return out
 
 
+def small_scene_leg() -> dict:
+    """The CPU-runnable scene run in full: 128x128 field and dye (4
+    Taylor-Green cells, disk dye), K = 4, dt = 1/60, nu = 0.1, 200 steps.  The
+    GPU runs pfs_fluid_timestep per step (the rule's tile side), timed with
+    events after an untimed run on a copy; the reference's src/fluid.cpp runs
+    the same 200 steps on one host core; the final states must be identical."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from probabilistic_fluid_simulation_amd import fluid, scenes
+
+    s = scenes.plain_16k()
+    n = s.steps
+
+    def on_gpu(sc):
+        bufs = [fluid.VPField.from_numpy(a) for a in (sc.vp, sc.tmp, sc.image, sc.itmp)]
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(n):
+            fluid.fluid_timestep(*bufs, sc.dt, sc.visc, sc.iters)
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / n, [b.numpy() for b in bufs]
+
+    on_gpu(s.copy())  # untimed: first launches
+    gpu_ms, got = on_gpu(s.copy())
+    ref = oracle.ref(s.iters)
+    c = s.copy()
+    t0 = time.perf_counter()
+    if ref is not None:
+        kind = "reference src/fluid.cpp (g++ -O2 -ffp-contract=off), 1 thread"
+        want = ref.run(c.vp, c.tmp, c.image, c.itmp, c.dt, c.visc, n)
+    else:
+        kind = "C restatement of src/fluid.cpp, 1 thread"
+        orc = oracle.get(32)
+        orc.set_threads(1)
+        want = orc.run(c.vp, c.tmp, c.image, c.itmp, c.dt, c.visc, c.iters, n)
+        orc.set_threads(orc.max_threads())
+    cpu_s = time.perf_counter() - t0
+    same = all(np.asarray(g).tobytes() == np.asarray(w).tobytes() for g, w in zip(got, want))
+    return {"workload": s.name, "steps": n, "tile": fluid.get_step_tile(128, 128, s.iters, True),
+            "gpu_us_per_step": gpu_ms * 1e3, "gpu_cell_steps_per_s": s.cells / (gpu_ms * 1e-3),
+            "cpu_us_per_step": cpu_s / n * 1e6, "cpu_cell_steps_per_s": s.cells * n / cpu_s,
+            "cpu": kind, "bit_exact": bool(same)}
+
+
 def tiles_leg(reps: int = 50) -> dict:
     """The fused kernels' tile side over field sizes: pfs_autotune_step times
     a whole timestep (field and dye of one size, dt = 1/120, nu = 0.1) for
@@ -552,6 +600,7 @@ def main() -> int:
         out["batched"] = batched_leg(args.batched_scenes, args.batched_steps)
     if not args.no_tiles:
         out["tiles"] = tiles_leg()
+        out["small_scene"] = small_scene_leg()
     if not args.no_served:
         out["served"] = [served_leg(float(r), args.served_seconds)
                          for r in args.served_rates.split(",")]
