@@ -125,8 +125,10 @@ def small_scene_leg() -> dict:
     """The CPU-runnable scene run in full: 128x128 field and dye (4
     Taylor-Green cells, disk dye), K = 4, dt = 1/60, nu = 0.1, 200 steps.  The
     GPU runs pfs_fluid_timestep per step (the rule's tile side), timed with
-    events after an untimed run on a copy; the reference's src/fluid.cpp runs
-    the same 200 steps on one host core; the final states must be identical."""
+    events after an untimed run on a copy, and pfs_fluid_run (the steps
+    replayed from a hipGraph) for 200 more; the reference's src/fluid.cpp
+    runs the same steps on one host core; the final states must be
+    identical."""
     import numpy as np
     import torch
 
@@ -136,19 +138,29 @@ def small_scene_leg() -> dict:
     s = scenes.plain_16k()
     n = s.steps
 
-    def on_gpu(sc):
-        bufs = [fluid.VPField.from_numpy(a) for a in (sc.vp, sc.tmp, sc.image, sc.itmp)]
+    def on_gpu(bufs, run):
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
-        for _ in range(n):
-            fluid.fluid_timestep(*bufs, sc.dt, sc.visc, sc.iters)
+        if run:
+            fluid.fluid_run(*bufs, s.dt, s.visc, s.iters, n)
+        else:
+            for _ in range(n):
+                fluid.fluid_timestep(*bufs, s.dt, s.visc, s.iters)
         e1.record()
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / n, [b.numpy() for b in bufs]
 
-    on_gpu(s.copy())  # untimed: first launches
-    gpu_ms, got = on_gpu(s.copy())
+    def dev(sc):
+        return [fluid.VPField.from_numpy(a) for a in (sc.vp, sc.tmp, sc.image, sc.itmp)]
+
+    on_gpu(dev(s.copy()), False)  # untimed: first launches
+    gpu_ms, got = on_gpu(dev(s.copy()), False)
+    # pfs_fluid_run: the untimed first call captures the graph of these
+    # buffers; the timed second call replays it, steps n+1 .. 2n.
+    bufs = dev(s.copy())
+    on_gpu(bufs, True)
+    run_ms, got_run = on_gpu(bufs, True)
     ref = oracle.ref(s.iters)
     c = s.copy()
     t0 = time.perf_counter()
@@ -162,9 +174,15 @@ def small_scene_leg() -> dict:
         want = orc.run(c.vp, c.tmp, c.image, c.itmp, c.dt, c.visc, c.iters, n)
         orc.set_threads(orc.max_threads())
     cpu_s = time.perf_counter() - t0
+    c = s.copy()
+    want_run = (ref.run(c.vp, c.tmp, c.image, c.itmp, c.dt, c.visc, 2 * n) if ref is not None
+                else oracle.get(32).run(c.vp, c.tmp, c.image, c.itmp, c.dt, c.visc, c.iters, 2 * n))
+    run_same = all(np.asarray(g).tobytes() == np.asarray(w).tobytes()
+                   for g, w in zip(got_run, want_run))
     same = all(np.asarray(g).tobytes() == np.asarray(w).tobytes() for g, w in zip(got, want))
     return {"workload": s.name, "steps": n, "tile": fluid.get_step_tile(128, 128, s.iters, True),
             "gpu_us_per_step": gpu_ms * 1e3, "gpu_cell_steps_per_s": s.cells / (gpu_ms * 1e-3),
+            "gpu_run_us_per_step": run_ms * 1e3, "gpu_run_bit_exact": bool(run_same),
             "cpu_us_per_step": cpu_s / n * 1e6, "cpu_cell_steps_per_s": s.cells * n / cpu_s,
             "cpu": kind, "bit_exact": bool(same)}
 

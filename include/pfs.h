@@ -121,6 +121,16 @@ int pfs_simulate_fluid_step_dev(float **vp, float **tmp, float dt, float viscosi
 int pfs_advect_color_step_dev(float **image, float **itmp, float **vp, float dt, int ix, int iy,
                               int iz, int vx, int vy, int vz, void *stream);
 
+/* nsteps iterations of the reference driver loop (src/main.cpp:219-240) in
+ * one call: pfs_fluid_timestep nsteps times, results identical.  On the
+ * fused path runs of 16 and of 2 consecutive steps are captured once in
+ * hipGraphs (kept, up to 16 per device, for later calls with the same
+ * buffers, shapes, coefficients, tile side and stream workspace) and
+ * replayed, which saves the per-launch host cost that bounds small fields.
+ * image/itmp swap nsteps times, as the per-step calls would. */
+int pfs_fluid_run(pfs_field *vp, pfs_field *tmp, pfs_field *image, pfs_field *itmp, float dt,
+                  float viscosity, int iters, int nsteps, void *stream);
+
 /* Step-path selection for pfs_simulate_fluid_step / _dev (process-wide).
  *   PFS_PATH_AUTO     fused kernels when applicable (default)
  *   PFS_PATH_PER_OP   always the per-operator sequence (the reference's

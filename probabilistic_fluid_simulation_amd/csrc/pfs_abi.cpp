@@ -7,6 +7,7 @@
 #include <atomic>
 #include <climits>
 #include <cstdio>
+#include <deque>
 #include <map>
 #include <mutex>
 #include <string>
@@ -280,6 +281,132 @@ static int timestep(pfs_field *vp, pfs_field *tmp, pfs_field *image, pfs_field *
     if (!image) return PFS_OK;
     if (!dye_fused) return pfs_advect_color_step(image, itmp, vp, dt, stream);
     std::swap(image->data, itmp->data);  // src/fluid.cpp:317-319
+    return PFS_OK;
+}
+
+// An even number of consecutive fused timesteps captured in a hipGraph, kept
+// for later calls with the same buffers, shapes, coefficients, tile side and
+// workspace.  (Even: the dye buffers swap every step, so an even count leaves
+// the roles as they were.)  Graphs are captured on a stream of the library's
+// own, since the caller's may be the null stream, which cannot capture; they
+// replay on the caller's stream.
+struct RunGraph {
+    int dev;
+    pfs_field f[4];
+    float dt, visc;
+    int iters, tile, steps;
+    float *ws;
+    hipGraphExec_t exec;
+    bool same(const RunGraph &o) const {
+        if (o.dev != dev || o.dt != dt || o.visc != visc || o.iters != iters || o.tile != tile ||
+            o.steps != steps || o.ws != ws)
+            return false;
+        for (int i = 0; i < 4; ++i)
+            if (f[i].x != o.f[i].x || f[i].y != o.f[i].y || f[i].z != o.f[i].z ||
+                f[i].data != o.f[i].data)
+                return false;
+        return true;
+    }
+};
+static std::mutex g_graph_mutex;
+static std::deque<RunGraph> g_graphs;         // most recently used last
+static std::map<int, hipStream_t> g_capture;  // per device
+constexpr int kMaxRunGraphs = 16;             // per device
+constexpr int kRunBlock = 16;                 // steps per graph of a long run
+
+// Records the launches of r.steps fused timesteps on `cs` (capturing).
+static int capture_steps(const RunGraph &r, bool dye_fused, hipStream_t cs) {
+    const pfs_field &vp = r.f[0], &tmp = r.f[1];
+    pfs_field im = r.f[2], itm = r.f[3];
+    for (int k = 0; k < r.steps; ++k) {
+        StepArgs a{vp.data, tmp.data, dye_fused ? im.data : nullptr,
+                   dye_fused ? itm.data : nullptr, vp.x, vp.y, r.dt, r.visc, r.iters, r.tile};
+        PFS_TRY(fused_step(a, r.ws, cs, nullptr, 1));
+        if (!dye_fused) PFS_TRY(pfs_advect_color(&im, &itm, &vp, r.dt, cs));
+        std::swap(im.data, itm.data);
+    }
+    return PFS_OK;
+}
+
+// The cached graph equal to `want`, captured now if there is none.  Called
+// with g_graph_mutex held.
+static int run_graph(RunGraph want, bool dye_fused, hipGraphExec_t *out) {
+    for (auto it = g_graphs.begin(); it != g_graphs.end(); ++it)
+        if (it->same(want)) {
+            const RunGraph r = *it;
+            g_graphs.erase(it);
+            g_graphs.push_back(r);
+            *out = r.exec;
+            return PFS_OK;
+        }
+    hipStream_t &cs = g_capture[want.dev];
+    if (!cs) PFS_HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    PFS_HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    int rc = capture_steps(want, dye_fused, cs);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(cs, &g);
+    if (rc == PFS_OK && e != hipSuccess)
+        rc = set_error(PFS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    if (rc == PFS_OK && hipGraphInstantiate(&want.exec, g, nullptr, nullptr, 0) != hipSuccess)
+        rc = set_error(PFS_ERR_HIP, "hipGraphInstantiate failed");
+    if (g) (void)hipGraphDestroy(g);
+    if (rc != PFS_OK) return rc;
+    int held = 0;
+    for (const RunGraph &r : g_graphs) held += r.dev == want.dev;
+    if (held >= kMaxRunGraphs) {
+        // Evict this device's least recently used graph once nothing queued
+        // on the device can still be running it.
+        PFS_HIP_TRY(hipDeviceSynchronize());
+        for (auto it = g_graphs.begin(); it != g_graphs.end(); ++it)
+            if (it->dev == want.dev) {
+                (void)hipGraphExecDestroy(it->exec);
+                g_graphs.erase(it);
+                break;
+            }
+    }
+    g_graphs.push_back(want);
+    *out = want.exec;
+    return PFS_OK;
+}
+
+int pfs_fluid_run(pfs_field *vp, pfs_field *tmp, pfs_field *image, pfs_field *itmp, float dt,
+                  float viscosity, int iters, int nsteps, void *stream) {
+    if (!image || !itmp) return set_error(PFS_ERR_INVALID, "image or itmp is NULL");
+    if (nsteps < 0) return set_error(PFS_ERR_INVALID, "nsteps < 0");
+    PFS_TRY(check_step_fields(vp, tmp, iters));
+    if (nsteps < 4 || !fused_ok(vp, tmp, iters)) {
+        for (int i = 0; i < nsteps; ++i)
+            PFS_TRY(timestep(vp, tmp, image, itmp, dt, viscosity, iters, stream, nullptr));
+        return PFS_OK;
+    }
+    // One eager step checks the arguments and sizes the stream's workspace
+    // outside any capture; an odd count takes a second one.
+    PFS_TRY(timestep(vp, tmp, image, itmp, dt, viscosity, iters, stream, nullptr));
+    int left = nsteps - 1;
+    if (left % 2) {
+        PFS_TRY(timestep(vp, tmp, image, itmp, dt, viscosity, iters, stream, nullptr));
+        --left;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    RunGraph want{0, {*vp, *tmp, *image, *itmp}, dt, viscosity, iters, 0, 0, nullptr, nullptr};
+    PFS_HIP_TRY(hipGetDevice(&want.dev));
+    PFS_TRY(get_workspace(fused_workspace_bytes(vp->x, vp->y), s, &want.ws));  // allocated already
+    const bool dye_fused = image->x == vp->x && image->y == vp->y && image->z == 4 &&
+                           aligned16(image->data) && aligned16(itmp->data);
+    want.tile = step_tile(vp->x, vp->y, iters, dye_fused);
+
+    std::lock_guard<std::mutex> lock(g_graph_mutex);
+    hipGraphExec_t exec = nullptr;
+    if (left >= kRunBlock) {  // blocks of kRunBlock steps, then pairs
+        want.steps = kRunBlock;
+        PFS_TRY(run_graph(want, dye_fused, &exec));
+        for (; left >= kRunBlock; left -= kRunBlock) PFS_HIP_TRY(hipGraphLaunch(exec, s));
+    }
+    if (left) {
+        want.steps = 2;
+        PFS_TRY(run_graph(want, dye_fused, &exec));
+        for (; left; left -= 2) PFS_HIP_TRY(hipGraphLaunch(exec, s));
+    }
     return PFS_OK;
 }
 

@@ -161,6 +161,17 @@ def fluid_timestep(vp: VPField, tmp: VPField, image: VPField, itmp: VPField, dt:
     b.sync_back()
 
 
+def fluid_run(vp: VPField, tmp: VPField, image: VPField, itmp: VPField, dt: float,
+              viscosity: float, iters: int, nsteps: int, stream=None) -> None:
+    """nsteps timesteps in one call (fluid_timestep nsteps times, identical
+    results), replayed from hipGraphs of 16 and 2 steps kept for these
+    buffers.  Swaps image.data and itmp.data nsteps times."""
+    b = _Bound(vp, tmp, image, itmp)
+    _lib.call("pfs_fluid_run", b.ptr(0), b.ptr(1), b.ptr(2), b.ptr(3), dt, viscosity, iters,
+              nsteps, _stream(stream))
+    b.sync_back()
+
+
 _precise_ws = {}
 
 

@@ -140,6 +140,69 @@ def test_fluid_timestep_combined(orc, shape, k):
     assert_bitexact(gpu_run(s, 3, combined=True), oracle_run(orc, s, 3))
 
 
+RUN_SHAPES = [(8, 8, 8, 8), (130, 67, 130, 67), (64, 48, 96, 64), (17, 13, 23, 19)]
+
+
+def _run(s, calls, stream=None):
+    """fluid_run once per entry of `calls` (step counts), same buffers."""
+    vp, tmp, im, itmp = to_dev(s)
+    for n in calls:
+        fluid.fluid_run(vp, tmp, im, itmp, s.dt, s.visc, s.iters, n, stream=stream)
+    torch.cuda.synchronize()
+    return tuple(f.numpy() for f in (vp, tmp, im, itmp))
+
+
+@pytest.mark.parametrize("calls", [(0,), (1,), (3,), (4,), (7,), (10,), (5, 6, 4), (36, 19, 18)],
+                         ids=lambda c: "+".join(map(str, c)))
+@pytest.mark.parametrize("shape", RUN_SHAPES, ids=_ids(RUN_SHAPES))
+def test_fluid_run_equals_the_step_loop(orc, shape, calls):
+    """fluid_run(n) equals n driver-loop timesteps of fluid.cpp: odd and even
+    counts, counts below the graph threshold, runs long enough for the
+    16-step graph plus 2-step graphs, and repeated calls on the same buffers
+    that replay the graphs kept from the first."""
+    s = scenes.random_scene(*shape, seed=41, dt=0.7, visc=0.25, iters=8, vel_scale=10.0)
+    assert_bitexact(_run(s, calls), oracle_run(orc, s, sum(calls)))
+
+
+@pytest.mark.parametrize("k", [1, 9])
+def test_fluid_run_on_a_stream_and_per_op(orc, k):
+    s = scenes.random_scene(100, 70, 100, 70, seed=43, dt=0.6, visc=0.2, iters=k)
+    want = oracle_run(orc, s, 9)
+    st = torch.cuda.Stream()
+    with torch.cuda.stream(st):
+        assert_bitexact(_run(s, (5, 4), stream=st), want)
+    fluid.set_step_path("per_op")  # no graph: the per-operator loop
+    assert_bitexact(_run(s, (5, 4)), want)
+
+
+def test_fluid_run_after_the_workspace_grows(orc):
+    """A larger field on the same stream regrows the step workspace between
+    two runs of a small one: the small field's next run must not replay a
+    graph that points at the freed block."""
+    a = scenes.random_scene(96, 96, 96, 96, seed=44, dt=0.6, visc=0.2, iters=8)
+    big = scenes.random_scene(700, 600, 700, 600, seed=45, dt=0.6, visc=0.2, iters=8)
+    bufs = to_dev(a)
+    fluid.fluid_run(*bufs, a.dt, a.visc, 8, 6)
+    fluid.fluid_run(*to_dev(big), big.dt, big.visc, 8, 4)
+    fluid.fluid_run(*bufs, a.dt, a.visc, 8, 6)
+    torch.cuda.synchronize()
+    assert_bitexact(tuple(f.numpy() for f in bufs), oracle_run(orc, a, 12))
+
+
+def test_fluid_run_evicts_old_graphs(orc):
+    """More distinct runs than the graph cache holds (one per time step dt):
+    every run stays exact, and the first configuration, evicted by then, is
+    captured again on its next call."""
+    s = scenes.random_scene(40, 30, 40, 30, seed=46, dt=0.5, visc=0.2, iters=4)
+    bufs = to_dev(s)
+    want = (s.vp, s.tmp, s.image, s.itmp)
+    for dt in [0.5 + 0.01 * i for i in range(20)] + [0.5]:
+        fluid.fluid_run(*bufs, dt, 0.2, 4, 4)
+        want = orc.run(*(w.copy() for w in want), dt, 0.2, 4, 4)
+    torch.cuda.synchronize()
+    assert_bitexact(tuple(f.numpy() for f in bufs), want)
+
+
 @pytest.mark.parametrize("shape", [(256, 192, 256, 192), (200, 333, 400, 111)])
 @pytest.mark.parametrize("k", [4, 7, 8, 12])
 def test_steps_multi_tile(orc, shape, k):
