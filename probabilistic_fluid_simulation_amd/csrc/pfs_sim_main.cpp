@@ -1,12 +1,14 @@
 // The simulation driver of the reference (src/main.cpp:71-280) on the GPU:
 // the same command line, the same input files and messages, every step one
-// pfs_fluid_timestep (src/main.cpp:222-225), every frame <dir>/<step>.png.
+// timestep of src/main.cpp:222-225, every frame <dir>/<step>.png.  The steps
+// up to the next written frame (all of them without frames) are one
+// pfs_fluid_run call, replayed from hipGraphs when there are enough.
 //
 // Frames leave the device without stalling the steps: after step i the
 // current image is copied on a second stream into one of a ring of pinned
-// host buffers, and a writer thread encodes it while the GPU runs on.  The
-// step that would overwrite an image buffer (image and itmp swap every
-// step) first waits, on the device, for the copy that reads it.
+// host buffers, and a writer thread encodes it while the GPU runs on.  Steps
+// that would overwrite an image buffer (image and itmp swap every step)
+// first wait, on the device, for the copy that reads it.
 //
 // Besides the reference's arguments:
 //   --iters K     Jacobi sweeps per solve (the reference's compile-time
@@ -23,6 +25,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstdio>
@@ -344,16 +347,23 @@ int run(int argc, char **argv) {
     std::cout << "Simulating [" << vp.h << " x " << vp.w << "] domain for " << n_timesteps
               << " timesteps at dt=" << delta_t << "..." << std::endl;
     const auto t0 = std::chrono::high_resolution_clock::now();
-    for (int i = 0; i < n_timesteps; ++i) {
-        // This step writes itmp: wait for the copy still reading it.
-        const int target = buffer_id(f_itmp.data);
-        if (reader_of[target] >= 0) {
-            SIM_HIP(hipStreamWaitEvent(dev.comp, wr.copied[reader_of[target]], 0));
-            reader_of[target] = -1;
-        }
-        SIM_PFS(pfs_fluid_timestep(&f_vp, &f_tmp, &f_img, &f_itmp, delta_t, viscosity, iters,
-                                   dev.comp));
-        if (timing_only || i % every != 0) continue;
+    for (int i = 0; i < n_timesteps;) {
+        // Steps i..last run in one pfs_fluid_run call (hipGraph replay when
+        // long): up to the next written frame, or to the end.
+        int last = n_timesteps - 1;
+        if (!timing_only) last = std::min(last, (i + every - 1) / every * every);
+        const int count = last - i + 1;
+        // The steps write itmp (one step) or both image buffers (more): wait
+        // for the copies still reading them.
+        for (int target = 0; target < 2; ++target)
+            if (reader_of[target] >= 0 && (count > 1 || target == buffer_id(f_itmp.data))) {
+                SIM_HIP(hipStreamWaitEvent(dev.comp, wr.copied[reader_of[target]], 0));
+                reader_of[target] = -1;
+            }
+        SIM_PFS(pfs_fluid_run(&f_vp, &f_tmp, &f_img, &f_itmp, delta_t, viscosity, iters, count,
+                              dev.comp));
+        i = last + 1;
+        if (timing_only || last % every != 0) continue;
         const int slot = wr.acquire();
         if (reader_of[0] == slot) reader_of[0] = -1;
         if (reader_of[1] == slot) reader_of[1] = -1;
@@ -362,7 +372,7 @@ int run(int argc, char **argv) {
         SIM_HIP(hipMemcpyAsync(wr.host[slot], f_img.data, ibytes, hipMemcpyDeviceToHost, dev.copy));
         SIM_HIP(hipEventRecord(wr.copied[slot], dev.copy));
         reader_of[buffer_id(f_img.data)] = slot;
-        wr.submit(slot, i);
+        wr.submit(slot, last);
     }
     SIM_HIP(hipStreamSynchronize(dev.comp));
     wr.finish();

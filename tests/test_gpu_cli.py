@@ -90,6 +90,28 @@ def test_npy_frames_every_and_per_op(orc, tmp_path):
             assert g.tobytes() == w.tobytes()
 
 
+def test_long_runs_between_frames(orc, tmp_path):
+    """Frames every 9th step over 40 steps: the steps between frames run as
+    one pfs_fluid_run each (graph replay), the written frames and the final
+    state stay those of the step loop."""
+    pi, pv = _inputs(tmp_path, 48, 40, 48, 40, seed=3)
+    steps, k = 40, 4
+    scene = pngio.load_scene(pi, pv, np.float32(0.6), np.float32(0.2), iters=k)
+    frames, final = _oracle_frames(orc, scene, steps)
+    out, dump = tmp_path / "frames", tmp_path / "dump"
+    out.mkdir()
+    dump.mkdir()
+    assert sim_main([str(steps), "0.6", "0.2", pi, pv, str(out), "--iters", str(k), "--npy",
+                     "--every", "9", "--dump", str(dump), "--quiet", "--buffers", "2"]) == 0
+    assert sorted(os.listdir(out), key=lambda f: int(f[:-4])) == [
+        f"{i}.npy" for i in (0, 9, 18, 27, 36)]
+    for i in (0, 9, 18, 27, 36):
+        got = np.load(str(out / f"{i}.npy"), allow_pickle=False)
+        assert got.tobytes() == frames[i].tobytes(), i
+    for g, w in zip(_load_dump(str(dump)), final):
+        assert g.tobytes() == w.tobytes()
+
+
 def test_timing_mode_and_npy_inputs(orc, tmp_path):
     """Five arguments: no frames (src/main.cpp:108-111).  .npy inputs are the
     raw float fields (the velocity already rescaled)."""
@@ -101,8 +123,9 @@ def test_timing_mode_and_npy_inputs(orc, tmp_path):
     assert sim_main(["3", "0.5", "0.1", pi, pv, "--iters", "2"]) == 0
     dump = tmp_path / "d"
     dump.mkdir()
-    assert sim_main(["3", "0.5", "0.1", pi, pv, "--iters", "2", "--dump", str(dump)]) == 0
-    _, final = _oracle_frames(orc, scene, 3)
+    # 23 steps without frames: one pfs_fluid_run of all of them.
+    assert sim_main(["23", "0.5", "0.1", pi, pv, "--iters", "2", "--dump", str(dump)]) == 0
+    _, final = _oracle_frames(orc, scene, 23)
     for g, w in zip(_load_dump(str(dump)), final):
         assert g.tobytes() == w.tobytes()
 
