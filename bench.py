@@ -499,6 +499,7 @@ def main() -> int:
     ap.add_argument("--batched-steps", type=int, default=300)
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-tiles", action="store_true")
+    ap.add_argument("--no-small-scene", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-particles", action="store_true")
     ap.add_argument("--particle-steps", type=int, default=50)
@@ -618,6 +619,7 @@ def main() -> int:
         out["batched"] = batched_leg(args.batched_scenes, args.batched_steps)
     if not args.no_tiles:
         out["tiles"] = tiles_leg()
+    if not args.no_small_scene:
         out["small_scene"] = small_scene_leg()
     if not args.no_served:
         out["served"] = [served_leg(float(r), args.served_seconds)
