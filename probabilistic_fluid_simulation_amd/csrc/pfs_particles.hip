@@ -464,8 +464,9 @@ __global__ void k_nbr_fill(NArgs a, const uint32_t *__restrict__ offsets, uint32
 // own list would otherwise go out as one scattered word per store).  A
 // block larger than the window takes further traversals, one per window.
 // Waves work independently: the LDS handoff is within one wave, whose LDS
-// accesses complete in order.
-constexpr int kFillWin = 2048;
+// accesses complete in order.  3072 slots hold a wave's lists at about 48
+// neighbours per particle in one traversal (48 KB per workgroup).
+constexpr int kFillWin = 3072;
 __global__ __launch_bounds__(256) void k_nbr_fill_lds(NArgs a, const uint32_t *__restrict__ offsets,
                                                       const uint32_t *__restrict__ counts,
                                                       uint32_t *__restrict__ nbr, size_t capacity) {
