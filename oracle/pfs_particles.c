@@ -76,8 +76,12 @@ int orcp_ngrid_build(const float *x, const float *y, int n, float kx, float ky, 
     return 0;
 }
 
-static float periodic(float d, float L, float half) {
-    return d > half ? d - L : (d < -half ? d + L : d);
+/* x_q - x_p on a periodic axis of length L (half = L/2): the nearer image,
+ * with the far operand moved by L before the subtraction (exact for a
+ * position in (L/2, L)), so that only nearby values are subtracted. */
+static float periodic_diff(float xq, float xp, float L, float half) {
+    const float d = xq - xp;
+    return d > half ? (xq - L) - xp : (d < -half ? xq - (xp - L) : d);
 }
 
 /* Neighbours of every particle (ids in traversal order), counts by particle
@@ -102,8 +106,8 @@ long long orcp_ngrid_neighbors(const float *xs, const float *ys, const uint32_t 
                 const uint32_t cc = (uint32_t)(ny * gx + nx);
                 for (uint32_t q = start[cc]; q < end[cc]; ++q) {
                     if ((int)q == p) continue;
-                    const float dx = periodic(xs[q] - px, W, half_w);
-                    const float dy = periodic(ys[q] - py, H, half_h);
+                    const float dx = periodic_diff(xs[q], px, W, half_w);
+                    const float dy = periodic_diff(ys[q], py, H, half_h);
                     const float d2 = dx * dx + dy * dy;
                     if (d2 < r2) {
                         if (nbr && offsets) nbr[o + k] = order ? order[q] : q;
@@ -139,8 +143,8 @@ void orcp_ngrid_density(const float *xs, const float *ys, const uint32_t *order,
                 const int nx = (cx + ox + gx) % gx;
                 const uint32_t cc = (uint32_t)(ny * gx + nx);
                 for (uint32_t q = start[cc]; q < end[cc]; ++q) {
-                    const float dx = periodic(xs[q] - px, W, half_w);
-                    const float dy = periodic(ys[q] - py, H, half_h);
+                    const float dx = periodic_diff(xs[q], px, W, half_w);
+                    const float dy = periodic_diff(ys[q], py, H, half_h);
                     const float d2 = dx * dx + dy * dy;
                     if (d2 < r2) {
                         const float t = r2 - d2;
@@ -189,8 +193,8 @@ int orcp_pbf_solve(float *xs, float *ys, int n, const uint32_t *offsets, const u
             float S = h2 * h2 * h2, gsx = 0.0f, gsy = 0.0f, g2 = 0.0f;
             for (uint32_t k = 0; k < counts[p]; ++k) {
                 const uint32_t q = nbr[offsets[p] + k];
-                const float dx = periodic(xs[q] - xs[p], W, half_w);
-                const float dy = periodic(ys[q] - ys[p], H, half_h);
+                const float dx = periodic_diff(xs[q], xs[p], W, half_w);
+                const float dy = periodic_diff(ys[q], ys[p], H, half_h);
                 const float d2 = dx * dx + dy * dy;
                 if (d2 < h2) {
                     const float t2 = h2 - d2;
@@ -217,8 +221,8 @@ int orcp_pbf_solve(float *xs, float *ys, int n, const uint32_t *offsets, const u
             float ax = 0.0f, ay = 0.0f;
             for (uint32_t k = 0; k < counts[p]; ++k) {
                 const uint32_t q = nbr[offsets[p] + k];
-                const float dx = periodic(xs[q] - xs[p], W, half_w);
-                const float dy = periodic(ys[q] - ys[p], H, half_h);
+                const float dx = periodic_diff(xs[q], xs[p], W, half_w);
+                const float dy = periodic_diff(ys[q], ys[p], H, half_h);
                 const float w = spiky_w(h, h2, gc, dx * dx + dy * dy);
                 if (w != 0.0f) {
                     const float s = lp + lambda[q];

@@ -355,9 +355,14 @@ __global__ void k_gather_u32(const uint32_t *__restrict__ index, const uint32_t 
     out[p] = i < (uint32_t)n ? in[i] : 0u;
 }
 
-// Periodic displacement on an axis of length L (half = L/2).
-__device__ __forceinline__ float periodic(float d, float L, float half) {
-    return d > half ? d - L : (d < -half ? d + L : d);
+// x_q - x_p on a periodic axis of length L (half = L/2), positions in
+// [0, L): the nearer image.  Across the wrap the far operand is moved by L
+// first, exactly (it lies in (L/2, L), so x - L is exact by Sterbenz), and
+// the two nearby values are then subtracted with one rounding at their own
+// scale; subtracting first would round at L's scale.
+__device__ __forceinline__ float periodic_diff(float xq, float xp, float L, float half) {
+    const float d = xq - xp;
+    return d > half ? (xq - L) - xp : (d < -half ? xq - (xp - L) : d);
 }
 
 struct NArgs {
@@ -426,8 +431,8 @@ __device__ __forceinline__ void for_neighbors(const NArgs &a, int p, F &&f) {
         });
     } else {
         rows([&](float ox, float oy) {
-            const float dx = periodic(ox - px, a.W, a.half_w);
-            const float dy = periodic(oy - py, a.H, a.half_h);
+            const float dx = periodic_diff(ox, px, a.W, a.half_w);
+            const float dy = periodic_diff(oy, py, a.H, a.half_h);
             return dx * dx + dy * dy;
         });
     }
@@ -667,8 +672,8 @@ __device__ __forceinline__ void for_listed(const PbfArgs &b, const Staged &st, i
     const uint32_t o = b.noff[p], c = b.ncnt[p];
     const bool on = st.on;
     auto one = [&](uint32_t q, uint32_t slot, float ox, float oy) {
-        const float dx = periodic(ox - px, b.W, b.half_w);
-        const float dy = periodic(oy - py, b.H, b.half_h);
+        const float dx = periodic_diff(ox, px, b.W, b.half_w);
+        const float dy = periodic_diff(oy, py, b.H, b.half_h);
         f(q, slot, dx, dy, dx * dx + dy * dy);
     };
     constexpr int kB = 8;  // neighbours whose loads are in flight together

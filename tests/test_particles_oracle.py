@@ -65,11 +65,14 @@ def _brute_neighbors(x, y, W, H, r):
     """Neighbour sets by testing every pair with the same float32 periodic
     distance as the grid code."""
     f = np.float32
-    dx = x[None, :] - x[:, None]
-    dy = y[None, :] - y[:, None]
-    hw, hh = f(W) * f(0.5), f(H) * f(0.5)
-    dx = np.where(dx > hw, dx - f(W), np.where(dx < -hw, dx + f(W), dx))
-    dy = np.where(dy > hh, dy - f(H), np.where(dy < -hh, dy + f(H), dy))
+
+    def diff(v, L, half):  # v_q - v_p, the far operand moved by L first
+        q, p = v[None, :], v[:, None]
+        d = q - p
+        return np.where(d > half, (q - f(L)) - p, np.where(d < -half, q - (p - f(L)), d))
+
+    dx = diff(x, W, f(W) * f(0.5))
+    dy = diff(y, H, f(H) * f(0.5))
     d2 = dx * dx + dy * dy
     m = d2 < f(r) * f(r)
     np.fill_diagonal(m, False)
@@ -131,6 +134,40 @@ def test_density_against_fp64_brute_force():
         assert (np.abs(rho - want) <= bound).all(), np.max(np.abs(rho - want) / want)
         # ... and a dropped self term or neighbour would be far outside it
         assert (np.abs(rho - coef * r2 ** 3 - want) > bound).all()
+
+
+def test_density_across_the_wrap_against_exact_distances():
+    """Particles around a corner of a 4096 x 4096 domain, every list crossing
+    the wrap: the density is within (m + 6) u of an FP64 sum over the exact
+    distances of the float32 positions.  The far position is moved by the
+    domain length before subtracting; subtracting first (x_q - x_p, then
+    +-L) rounds at the domain's scale (ulp(4096) = 2^-11) and misses this
+    bound by 24x."""
+    f = np.float32
+    W = H = 4096.0
+    r = 1.5
+    rng = np.random.default_rng(5)
+    n = 600
+    x = (rng.random(n) * 6 - 3) % W
+    y = (rng.random(n) * 6 - 3) % H
+    x = np.where(x.astype(f) >= f(W), f(0), x.astype(f))
+    y = np.where(y.astype(f) >= f(H), f(0), y.astype(f))
+    g, og = _oracle(x, y, W, H, r)
+    rho = og.density(1.0).astype(np.float64)
+    counts, offsets, nbr = og.neighbors()
+    x64, y64 = x.astype(np.float64), y.astype(np.float64)
+
+    def d64(a, b, L):  # exact in float64
+        d = a - b
+        return np.where(d > L / 2, d - L, np.where(d < -L / 2, d + L, d))
+
+    r2 = float(f(r) * f(r))
+    coef = 4.0 / (np.pi * float(f(r)) ** 8)
+    for i in range(n):
+        js = nbr[offsets[i]:offsets[i] + counts[i]].astype(np.int64)
+        dx, dy = d64(x64[js], x64[i], W), d64(y64[js], y64[i], H)
+        want = coef * (r2 ** 3 + np.sum((r2 - (dx * dx + dy * dy)) ** 3))
+        assert abs(rho[i] - want) <= (counts[i] + 7) * 2.0 ** -24 * want, i
 
 
 def test_tracers_integer_shift_is_exact():
