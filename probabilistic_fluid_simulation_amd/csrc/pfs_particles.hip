@@ -362,7 +362,8 @@ __global__ void k_gather_u32(const uint32_t *__restrict__ index, const uint32_t 
 // scale; subtracting first would round at L's scale.
 __device__ __forceinline__ float periodic_diff(float xq, float xp, float L, float half) {
     const float d = xq - xp;
-    return d > half ? (xq - L) - xp : (d < -half ? xq - (xp - L) : d);
+    const float over = (xq - L) - xp, under = xq - (xp - L);  // both formed: selects, no branches
+    return d > half ? over : (d < -half ? under : d);
 }
 
 struct NArgs {
@@ -422,13 +423,43 @@ __device__ __forceinline__ void for_neighbors(const NArgs &a, int p, F &&f) {
             }
         }
     };
-    const bool interior = cx >= 1 && cx <= a.gx - 2 && cy >= 1 && cy <= a.gy - 2 && a.gx >= 7 &&
-                          a.gy >= 7;
+    const bool big = a.gx >= 7 && a.gy >= 7;
+    const bool interior = cx >= 1 && cx <= a.gx - 2 && cy >= 1 && cy <= a.gy - 2 && big;
     if (interior) {
         rows([&](float ox, float oy) {
             const float dx = ox - px, dy = oy - py;
             return dx * dx + dy * dy;
         });
+    } else if (big) {
+        // Near an edge of a grid of 7 or more cells a side, whether a
+        // candidate lies across the wrap is known per range: cell gx-1
+        // seen from cell 0 (its candidates lie in (W/2, W): shifted by -W),
+        // cell 0 seen from cell gx-1 (p lies there: p shifted by -W), and
+        // the same for rows.  Every other candidate is within 2W/7 of p and
+        // every shifted one at least 5W/7 away before the shift, so these
+        // are exactly the cases periodic_diff corrects, with its values:
+        // (x_q + -W) - x_p and (x_q + -0) - (x_p - W).
+        for (int oy = -1; oy <= 1; ++oy) {
+            const int ny = cy + oy < 0 ? cy + oy + a.gy : (cy + oy >= a.gy ? cy + oy - a.gy : cy + oy);
+            const uint32_t row = (uint32_t)(ny * a.gx);
+            const float qsy = (cy == 0 && oy == -1) ? -a.H : -0.0f;
+            const float pyu = (cy == a.gy - 1 && oy == 1) ? py - a.H : py;
+            auto shifted = [&](float qsx, float pxu) {
+                return [=](float ox, float oy_) {
+                    const float dx = (ox + qsx) - pxu, dy = (oy_ + qsy) - pyu;
+                    return dx * dx + dy * dy;
+                };
+            };
+            if (cx == 0) {  // cells gx-1 | 0, 1
+                scan(a.start[row + a.gx - 1], a.end[row + a.gx - 1], shifted(-a.W, px));
+                scan(a.start[row], a.end[row + 1], shifted(-0.0f, px));
+            } else if (cx == a.gx - 1) {  // cells gx-2, gx-1 | 0
+                scan(a.start[row + cx - 1], a.end[row + cx], shifted(-0.0f, px));
+                scan(a.start[row], a.end[row], shifted(-0.0f, px - a.W));
+            } else {
+                scan(a.start[row + cx - 1], a.end[row + cx + 1], shifted(-0.0f, px));
+            }
+        }
     } else {
         rows([&](float ox, float oy) {
             const float dx = periodic_diff(ox, px, a.W, a.half_w);
