@@ -1,0 +1,178 @@
+"""The reference's C++ interface (include/fluid.hpp, implemented in libpfs.so
+by csrc/fluid_shim.cpp), called the way a main.cpp-style caller links it: by
+its C++ symbols.  The vp_field flavour is called with host buffers, which
+the shim stages to the GPU and back; the float** flavour with device
+pointers.  Results are compared bit for bit with the oracle, and, at the
+reference's own NUM_JACOBI_ITERS = 30, with the reference's fluid.cpp where
+oracle/_ref holds its build."""
+from __future__ import annotations
+
+import ctypes
+import subprocess
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+import oracle  # noqa: E402
+from probabilistic_fluid_simulation_amd import _lib, scenes  # noqa: E402
+
+from gpu_util import assert_bitexact, oracle_run  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+class VpField(ctypes.Structure):  # includes/fluid.hpp:17-22
+    _fields_ = [("x", ctypes.c_int), ("y", ctypes.c_int), ("z", ctypes.c_int),
+                ("data", ctypes.c_void_p)]
+
+
+P, F, I = ctypes.c_void_p, ctypes.c_float, ctypes.c_int
+SIGS = {  # demangled signature -> argument types
+    "advect(vp_field*,vp_field*,float)": [P, P, F],
+    "advect_color(vp_field*,vp_field*,vp_field*,float)": [P, P, P, F],
+    "diffuse(vp_field*,vp_field*,float,float)": [P, P, F, F],
+    "addForces(vp_field*,float*)": [P, P],
+    "computePressure(vp_field*,vp_field*,float)": [P, P, F],
+    "subtractPressureGradient(vp_field*,vp_field*,float)": [P, P, F],
+    "simulate_fluid_step(vp_field*,vp_field*,float,float)": [P, P, F, F],
+    "advect_color_step(vp_field*,vp_field*,vp_field*,float)": [P, P, P, F],
+    "simulate_fluid_step(float**,float**,float,float,int,int,int)": [P, P, F, F, I, I, I],
+    "advect_color_step(float**,float**,float**,float,int,int,int,int,int,int)":
+        [P, P, P, F, I, I, I, I, I, I],
+    "set_num_jacobi_iters(int)": [I],
+    "get_num_jacobi_iters()": [],
+}
+
+
+@pytest.fixture(scope="module")
+def hpp():
+    """The fluid.hpp functions of libpfs.so, looked up by C++ symbol."""
+    lib = _lib.load()
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    syms = [ln.split()[-1] for ln in out.splitlines() if " T " in ln]
+    dem = subprocess.run(["c++filt"], input="\n".join(syms), capture_output=True, text=True,
+                         check=True).stdout.splitlines()
+    table = {d.replace(" ", ""): s for s, d in zip(syms, dem)}
+    fns = {}
+    for sig, args in SIGS.items():
+        f = getattr(lib, table[sig])
+        f.argtypes = args
+        f.restype = ctypes.c_int if sig.startswith("get_") else None
+        fns[sig.split("(")[0] + ("_dev" if "float**" in sig else "")] = f
+    yield fns
+    fns["set_num_jacobi_iters"](30)
+
+
+def _host(arrays):
+    """vp_fields over numpy arrays, and a map from data pointer to array."""
+    fields = [VpField(a.shape[1], a.shape[0], a.shape[2], a.ctypes.data) for a in arrays]
+    return fields, {a.ctypes.data: a for a in arrays}
+
+
+def _ref(f):
+    return ctypes.byref(f)
+
+
+@pytest.mark.parametrize("shape", [(17, 13, 23, 19), (64, 48, 96, 64), (130, 67, 130, 67)],
+                         ids=["17x13-img23x19", "64x48-img96x64", "130x67"])
+def test_host_step_loop_of_main_cpp(hpp, orc, shape):
+    """src/main.cpp:236-239 with host buffers: simulate_fluid_step then
+    advect_color_step, three times, K = 8."""
+    hpp["set_num_jacobi_iters"](8)
+    assert hpp["get_num_jacobi_iters"]() == 8
+    s = scenes.random_scene(*shape, seed=61, dt=0.7, visc=0.25, iters=8, vel_scale=10.0)
+    bufs = [s.vp.copy(), s.tmp.copy(), s.image.copy(), s.itmp.copy()]
+    f, by_ptr = _host(bufs)
+    for _ in range(3):
+        hpp["simulate_fluid_step"](_ref(f[0]), _ref(f[1]), s.dt, s.visc)
+        hpp["advect_color_step"](_ref(f[2]), _ref(f[3]), _ref(f[0]), s.dt)
+    got = [by_ptr[x.data] for x in f]
+    assert got[0] is bufs[0] and got[1] is bufs[1]  # identities restored
+    assert_bitexact(got, oracle_run(orc, s, 3))
+
+
+def test_device_step_loop_of_main_cpp(hpp, orc):
+    """src/main.cpp:219-225 with device pointers: the float** flavour
+    leaves the result in *vp and swaps *image and *itmp."""
+    hpp["set_num_jacobi_iters"](8)
+    s = scenes.random_scene(100, 70, 50, 35, seed=62, dt=0.6, visc=0.2, iters=8, vel_scale=8.0)
+    dev = [torch.from_numpy(a.copy()).cuda() for a in (s.vp, s.tmp, s.image, s.itmp)]
+    by_ptr = {t.data_ptr(): t for t in dev}
+    ptr = [ctypes.c_void_p(t.data_ptr()) for t in dev]
+    vy, vx, vz = s.vp.shape
+    iy, ix, iz = s.image.shape
+    for _ in range(3):
+        hpp["simulate_fluid_step_dev"](_ref(ptr[0]), _ref(ptr[1]), s.dt, s.visc, vx, vy, vz)
+        hpp["advect_color_step_dev"](_ref(ptr[2]), _ref(ptr[3]), _ref(ptr[0]), s.dt, ix, iy, iz,
+                                     vx, vy, vz)
+    torch.cuda.synchronize()
+    got = [by_ptr[p.value].cpu().numpy() for p in ptr]
+    assert_bitexact(got, oracle_run(orc, s, 3))
+
+
+@pytest.mark.parametrize("k", [1, 4])
+def test_host_operators(hpp, orc, k):
+    """The per-operator functions with host buffers, against the oracle's
+    operators, with the pointer swaps of diffuse and computePressure."""
+    hpp["set_num_jacobi_iters"](k)
+    s = scenes.random_scene(33, 21, 40, 30, seed=63, dt=0.8, visc=0.3, iters=k)
+
+    want = s.tmp.copy()
+    orc.advect(s.vp.copy(), want, s.dt)
+    a, b = s.vp.copy(), s.tmp.copy()
+    f, _ = _host([a, b])
+    hpp["advect"](_ref(f[0]), _ref(f[1]), s.dt)
+    assert_bitexact([b], [want], ("out",))
+
+    want = np.zeros_like(s.image)
+    orc.advect_color(s.image.copy(), want, s.vp.copy(), s.dt)
+    im, out, vp = s.image.copy(), np.zeros_like(s.image), s.vp.copy()
+    f, _ = _host([im, out, vp])
+    hpp["advect_color"](_ref(f[0]), _ref(f[1]), _ref(f[2]), s.dt)
+    assert_bitexact([out], [want], ("out",))
+
+    for name, ref in (("diffuse", lambda x, y: orc.diffuse(x, y, s.visc, s.dt, k)),
+                      ("computePressure", lambda x, y: orc.compute_pressure(x, y, s.dt, k))):
+        wa, wb = ref(s.vp.copy(), s.tmp.copy())
+        a, b = s.vp.copy(), s.tmp.copy()
+        f, by_ptr = _host([a, b])
+        if name == "diffuse":
+            hpp[name](_ref(f[0]), _ref(f[1]), s.visc, s.dt)
+        else:
+            hpp[name](_ref(f[0]), _ref(f[1]), s.dt)
+        assert_bitexact([by_ptr[f[0].data], by_ptr[f[1].data]], [wa, wb], ("vp", "vp_out"))
+
+    want = s.tmp.copy()
+    orc.subtract_pressure_gradient(s.vp.copy(), want, s.dt)
+    a, b = s.vp.copy(), s.tmp.copy()
+    f, _ = _host([a, b])
+    hpp["subtractPressureGradient"](_ref(f[0]), _ref(f[1]), s.dt)
+    assert_bitexact([b], [want], ("out",))
+
+    a = s.vp.copy()
+    f, _ = _host([a])
+    hpp["addForces"](_ref(f[0]), None)
+    assert a.tobytes() == s.vp.tobytes()
+
+
+def test_reference_jacobi_count_against_fluid_cpp(hpp, orc):
+    """NUM_JACOBI_ITERS = 30, the reference's build: two host steps equal the
+    oracle at K = 30 and, where oracle/_ref holds it, the reference's own
+    src/fluid.cpp built with K = 30."""
+    hpp["set_num_jacobi_iters"](30)
+    s = scenes.random_scene(40, 24, 33, 19, seed=64, dt=0.5, visc=0.2, iters=30, vel_scale=6.0)
+    bufs = [s.vp.copy(), s.tmp.copy(), s.image.copy(), s.itmp.copy()]
+    f, by_ptr = _host(bufs)
+    for _ in range(2):
+        hpp["simulate_fluid_step"](_ref(f[0]), _ref(f[1]), s.dt, s.visc)
+        hpp["advect_color_step"](_ref(f[2]), _ref(f[3]), _ref(f[0]), s.dt)
+    got = [by_ptr[x.data] for x in f]
+    assert_bitexact(got, oracle_run(orc, s, 2))
+    ref = oracle.ref(30)
+    if ref is None:
+        pytest.skip("oracle/_ref has no K = 30 build of src/fluid.cpp")
+    c = s.copy()
+    assert_bitexact(got, ref.run(c.vp, c.tmp, c.image, c.itmp, c.dt, c.visc, 2))
