@@ -94,6 +94,20 @@ def test_invalid_arguments_rejected_before_device_work():
         _lib.PFS_ERR_UNSUPPORTED
     with pytest.raises(_lib.PfsError):
         _lib.call("pfs_set_step_path", 7)
+    # pfs_fluid_run: its own arguments, then each step's, before any launch
+    img, itmp = _lib.PfsField(4, 4, 4, 0x6000), _lib.PfsField(4, 4, 4, 0x7000)
+    fr = lib.pfs_fluid_run
+    assert fr(ctypes.byref(good), ctypes.byref(other), null, ctypes.byref(itmp), 0.1, 0.1, 4,
+              8, None) == _lib.PFS_ERR_INVALID
+    assert fr(ctypes.byref(good), ctypes.byref(other), ctypes.byref(img), ctypes.byref(itmp),
+              0.1, 0.1, 4, -1, None) == _lib.PFS_ERR_INVALID
+    assert fr(ctypes.byref(good), ctypes.byref(other), ctypes.byref(img), ctypes.byref(img),
+              0.1, 0.1, 4, 8, None) == _lib.PFS_ERR_INVALID
+    assert b"alias" in lib.pfs_last_error()
+    assert fr(ctypes.byref(bad), ctypes.byref(other), ctypes.byref(img), ctypes.byref(itmp),
+              0.1, 0.1, 4, 8, None) == _lib.PFS_ERR_INVALID
+    assert fr(ctypes.byref(good), ctypes.byref(other), ctypes.byref(img), ctypes.byref(itmp),
+              0.1, 0.1, 4, 0, None) == _lib.PFS_OK  # no steps: nothing to do
 
 
 def test_step_path_switch_and_workspace_size():
