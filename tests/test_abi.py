@@ -72,6 +72,27 @@ def test_every_fluid_hpp_function_is_exported():
         assert sig in exported, sig
 
 
+def test_fluid_hpp_symbol_table_of_the_gpu_test():
+    """tests/test_gpu_fluid_hpp.py calls the fluid.hpp functions by a fixed
+    table of C++ symbols (a GPU process must not start nm): every one is
+    exported, and together they demangle to exactly the declarations of
+    include/fluid.hpp."""
+    import ast
+
+    src = open(os.path.join(os.path.dirname(__file__), "test_gpu_fluid_hpp.py")).read()
+    table = next(n.value for n in ast.parse(src).body
+                 if isinstance(n, ast.Assign) and getattr(n.targets[0], "id", "") == "SYMBOLS")
+    syms = sorted(v.elts[0].value for v in table.values)
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    assert set(syms) <= exported, set(syms) - exported
+    dem = subprocess.run(["c++filt"], input="\n".join(syms), capture_output=True, text=True,
+                         check=True).stdout.splitlines()
+    declared = {f"{n}({','.join(t)})".replace(" ", "") for n, t in _declared_cpp_functions()}
+    assert {d.replace(" ", "") for d in dem} == declared
+
+
 def test_invalid_arguments_rejected_before_device_work():
     lib = _lib.load()
     null = ctypes.POINTER(_lib.PfsField)()

@@ -8,7 +8,6 @@ oracle/_ref holds its build."""
 from __future__ import annotations
 
 import ctypes
-import subprocess
 
 import numpy as np
 import pytest
@@ -29,20 +28,24 @@ class VpField(ctypes.Structure):  # includes/fluid.hpp:17-22
 
 
 P, F, I = ctypes.c_void_p, ctypes.c_float, ctypes.c_int
-SIGS = {  # demangled signature -> argument types
-    "advect(vp_field*,vp_field*,float)": [P, P, F],
-    "advect_color(vp_field*,vp_field*,vp_field*,float)": [P, P, P, F],
-    "diffuse(vp_field*,vp_field*,float,float)": [P, P, F, F],
-    "addForces(vp_field*,float*)": [P, P],
-    "computePressure(vp_field*,vp_field*,float)": [P, P, F],
-    "subtractPressureGradient(vp_field*,vp_field*,float)": [P, P, F],
-    "simulate_fluid_step(vp_field*,vp_field*,float,float)": [P, P, F, F],
-    "advect_color_step(vp_field*,vp_field*,vp_field*,float)": [P, P, P, F],
-    "simulate_fluid_step(float**,float**,float,float,int,int,int)": [P, P, F, F, I, I, I],
-    "advect_color_step(float**,float**,float**,float,int,int,int,int,int,int)":
-        [P, P, P, F, I, I, I, I, I, I],
-    "set_num_jacobi_iters(int)": [I],
-    "get_num_jacobi_iters()": [],
+# The fluid.hpp functions by their Itanium C++ symbols (what a recompiled
+# main.cpp links against), with their argument types.  The table is fixed
+# here because a GPU process must not start programs (nm, c++filt);
+# tests/test_abi.py checks it against the library's exports on the CPU.
+SYMBOLS = {
+    "advect": ("_Z6advectP8vp_fieldS0_f", [P, P, F]),
+    "advect_color": ("_Z12advect_colorP8vp_fieldS0_S0_f", [P, P, P, F]),
+    "diffuse": ("_Z7diffuseP8vp_fieldS0_ff", [P, P, F, F]),
+    "addForces": ("_Z9addForcesP8vp_fieldPf", [P, P]),
+    "computePressure": ("_Z15computePressureP8vp_fieldS0_f", [P, P, F]),
+    "subtractPressureGradient": ("_Z24subtractPressureGradientP8vp_fieldS0_f", [P, P, F]),
+    "simulate_fluid_step": ("_Z19simulate_fluid_stepP8vp_fieldS0_ff", [P, P, F, F]),
+    "advect_color_step": ("_Z17advect_color_stepP8vp_fieldS0_S0_f", [P, P, P, F]),
+    "simulate_fluid_step_dev": ("_Z19simulate_fluid_stepPPfS0_ffiii", [P, P, F, F, I, I, I]),
+    "advect_color_step_dev": ("_Z17advect_color_stepPPfS0_S0_fiiiiii",
+                              [P, P, P, F, I, I, I, I, I, I]),
+    "set_num_jacobi_iters": ("_Z20set_num_jacobi_itersi", [I]),
+    "get_num_jacobi_iters": ("_Z20get_num_jacobi_itersv", []),
 }
 
 
@@ -50,18 +53,12 @@ SIGS = {  # demangled signature -> argument types
 def hpp():
     """The fluid.hpp functions of libpfs.so, looked up by C++ symbol."""
     lib = _lib.load()
-    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True,
-                         text=True, check=True).stdout
-    syms = [ln.split()[-1] for ln in out.splitlines() if " T " in ln]
-    dem = subprocess.run(["c++filt"], input="\n".join(syms), capture_output=True, text=True,
-                         check=True).stdout.splitlines()
-    table = {d.replace(" ", ""): s for s, d in zip(syms, dem)}
     fns = {}
-    for sig, args in SIGS.items():
-        f = getattr(lib, table[sig])
+    for name, (sym, args) in SYMBOLS.items():
+        f = getattr(lib, sym)
         f.argtypes = args
-        f.restype = ctypes.c_int if sig.startswith("get_") else None
-        fns[sig.split("(")[0] + ("_dev" if "float**" in sig else "")] = f
+        f.restype = ctypes.c_int if name.startswith("get_") else None
+        fns[name] = f
     yield fns
     fns["set_num_jacobi_iters"](30)
 
