@@ -121,6 +121,23 @@ def cpu_baseline(scene, budget_steps: int) -> dict:
     return out
 
 
+def whole_run_check(scene, end_state, steps: int) -> dict:
+    """The timed run's end state (warm-up and timed steps from the workload's
+    inputs) against the C restatement of src/fluid.cpp stepping the same
+    inputs as often, on every host thread: bit for bit, every buffer."""
+    import oracle
+
+    orc = oracle.get(32)
+    orc.set_threads(orc.max_threads())
+    c = scene.copy()
+    t0 = time.perf_counter()
+    want = orc.run(c.vp, c.tmp, c.image, c.itmp, c.dt, c.visc, c.iters, steps)
+    same = all(g.cpu().numpy().tobytes() == w.tobytes() for g, w in zip(end_state, want))
+    return {"steps": steps, "bit_exact": bool(same), "buffers": "vp, tmp, image, itmp",
+            "checker": f"C restatement of src/fluid.cpp, {orc.max_threads()} threads, "
+                       f"{time.perf_counter() - t0:.1f} s"}
+
+
 def small_scene_leg() -> dict:
     """The CPU-runnable scene run in full: 128x128 field and dye (4
     Taylor-Green cells, disk dye), K = 4, dt = 1/60, nu = 0.1, 200 steps.  The
@@ -604,6 +621,7 @@ def main() -> int:
         timestep()
     wall, wall_ev = timed(timestep, args.steps)
     value = s.cells * args.steps / wall
+    end_state = [f.data.clone() for f in (vp, tmp, im, itmp)]  # after warmup + steps steps
     for _ in range(args.warmup):
         two_calls()
     wall_two, _ = timed(two_calls, args.steps)
@@ -668,6 +686,8 @@ def main() -> int:
         out["roofline"]["valu_insts_per_launch"] = valu
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scenes.plain_1m(steps=args.steps), args.cpu_steps)
+        out["cpu_baseline"]["whole_run_check"] = whole_run_check(s, end_state,
+                                                                 args.warmup + args.steps)
     del vp, tmp, im, itmp
     if not args.no_batched:
         out["batched"] = batched_leg(args.batched_scenes, args.batched_steps)

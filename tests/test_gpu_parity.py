@@ -294,6 +294,20 @@ def test_plain_1m_bitexact(orc, combined):
     assert_bitexact(gpu_run(s, 3, combined=combined), oracle_run(orc, s, 3))
 
 
+def test_plain_1m_whole_run_bitexact(orc):
+    """The 1024x1024, K = 8 workload run in full, 500 steps, by one
+    pfs_fluid_run call (graph replay): every buffer bit-exact to the oracle
+    stepping the same inputs 500 times (on every host thread)."""
+    s = scenes.plain_1m()
+    assert s.steps == 500
+    bufs = to_dev(s)
+    fluid.fluid_run(*bufs, s.dt, s.visc, s.iters, s.steps)
+    torch.cuda.synchronize()
+    got = tuple(f.numpy() for f in bufs)
+    orc.set_threads(orc.max_threads())
+    assert_bitexact(got, oracle_run(orc, s, s.steps))
+
+
 @pytest.mark.parametrize("iters", [16, 30])
 def test_plain_1m_many_sweeps_bitexact(orc, iters):
     """The 1024x1024 workload at K = 16 and at the reference's default K = 30
