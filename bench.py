@@ -296,11 +296,23 @@ def batched_leg(n_scenes: int, steps: int) -> dict:
     # batched one) and by the FP64 twin; max |x - x64| / max |x64| over u, v.
     errs = [fp64_step_error(x[0].data, x[1].data, j) for j, x in zip(jobs, inputs)]
     errs += [fp64_step_error(p[0].data, p[1].data, j) for j, p in zip(jobs, plain)]
+    # The first 8 scenes' end states against the C restatement of
+    # src/fluid.cpp stepping the same inputs (the batch equals the loop).
+    import oracle
+
+    orc = oracle.get(32)
+    orc.set_threads(orc.max_threads())
+    checked = []
+    for j, x, p in list(zip(jobs, inputs, plain))[:8]:
+        want = orc.run(*(f.data.cpu().numpy() for f in x), j.dt, j.visc, j.iters, steps)
+        checked.append(all(g.data.cpu().numpy().tobytes() == w.tobytes() for g, w in zip(p, want)))
     cells = sum(j.fw * j.fh for j in jobs)
     return {"scenes": n_scenes, "steps_per_scene": steps, "iters": 8, "cells": cells,
             "jobs_per_s": n_scenes / t_batch, "plain_loop_jobs_per_s": n_scenes / t_plain,
             "speedup_vs_plain_loop": t_plain / t_batch,
             "cell_steps_per_s": cells * steps / t_batch, "results_identical": bool(same),
+            "oracle_check": {"scenes": len(checked), "steps": steps,
+                             "bit_exact": bool(all(checked))},
             "velocity_rel_err_vs_fp64_max": max(errs),
             "fp64_check": "one step of each scene from its input and from its final state, "
                           "FP32 step vs the FP64 twin, max |x - x64| / max |x64| over u, v",
