@@ -34,7 +34,8 @@
 namespace pfs {
 namespace {
 
-constexpr int kCBX = 64, kCBY = 4;  // dye blocks: 64 x 4 pixels
+constexpr int kCBX = 64, kCBY = 4;  // dye threads per block: 64 x 4
+constexpr int kCRows = 4;           // pixels per thread, kCBY rows apart: 64 x 16 per block
 
 template <int H, bool LAST, int TS>
 __global__ __launch_bounds__((Geo<H, TS>::NT)) void k_diffuse_batch(
@@ -57,8 +58,8 @@ __global__ __launch_bounds__(kCBX *kCBY) void k_color_batch(const ColorArgs *__r
     const ColorArgs a = tab[block_scene[blockIdx.x]];
     const int blk = blockIdx.x - a.block_base;
     const int i = (blk % a.blocks_x) * kCBX + threadIdx.x % kCBX;
-    const int j = (blk / a.blocks_x) * kCBY + threadIdx.x / kCBX;
-    if (i < a.iw && j < a.ih) advect_color_pixel(a, i, j);
+    const int j = (blk / a.blocks_x) * (kCBY * kCRows) + threadIdx.x / kCBX;
+    advect_color_column<kCRows>(a, i, j, kCBY);
 }
 
 using DiffuseBatchFn = void (*)(const DiffuseArgs *, const int *, int);
@@ -303,7 +304,8 @@ struct pfs_batch {
                     a.block_base = (int)block_scene_h.size();
                     c[q].push_back(a);
                 }
-                const int nb = c[0].back().blocks_x * ((s.dye[0].y + kCBY - 1) / kCBY);
+                const int nb =
+                    c[0].back().blocks_x * ((s.dye[0].y + kCBY * kCRows - 1) / (kCBY * kCRows));
                 block_scene_h.insert(block_scene_h.end(), nb, (int)k);
             }
             nblocks = (int)block_scene_h.size();

@@ -168,6 +168,57 @@ __device__ __forceinline__ void advect_color_pixel(const ColorArgs &a, int i, in
     }
 }
 
+// R pixels of column i, rows j0, j0 + dj, ... (those inside the image): the
+// operations of advect_color_pixel for each, with every gather of the R
+// pixels issued before any of their results is computed, so that the
+// gathers of different pixels overlap.  Rows past the image load the last
+// row (in bounds) and store nothing.
+template <int R>
+__device__ __forceinline__ void advect_color_column(const ColorArgs &a, int i, int j0, int dj) {
+    if (i >= a.iw) return;
+    if (!a.vec4) {
+        for (int r = 0; r < R; ++r)
+            if (j0 + r * dj < a.ih) advect_color_pixel(a, i, j0 + r * dj);
+        return;
+    }
+    const int vi = min((int)((float)i * a.viw), a.vw - 1);
+    float u[R], v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int j = min(j0 + r * dj, a.ih - 1);
+        const int vj = min((int)((float)j * a.vih), a.vh - 1);
+        const int vb = (vj * a.vw + vi) * a.vc;
+        u[r] = a.vp[vb];
+        v[r] = a.vp[vb + 1];
+    }
+    Trace t[R];
+    float4 f[R][4];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int j = min(j0 + r * dj, a.ih - 1);
+        const float xp = (float)i - div_n(a.dt_viw * u[r], a.fiw, a.inv_iw, a.pow2_iw);  // :97
+        const float yp = (float)j - div_n(a.dt_vih * v[r], a.fih, a.inv_ih, a.pow2_ih);  // :98
+        t[r] = trace(xp, yp, a.iw, a.ih, a.fiw, a.fih);
+        const float4 *img = reinterpret_cast<const float4 *>(a.img);
+        f[r][0] = img[t[r].j0 * a.iw + t[r].i0];
+        f[r][1] = img[t[r].j0 * a.iw + t[r].i1];
+        f[r][2] = img[t[r].j1 * a.iw + t[r].i0];
+        f[r][3] = img[t[r].j1 * a.iw + t[r].i1];
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int j = j0 + r * dj;
+        if (j >= a.ih) continue;
+        const Trace &q = t[r];
+        float4 o;
+        o.x = bilerp(q.sx, q.sy, f[r][0].x, f[r][1].x, f[r][2].x, f[r][3].x);
+        o.y = bilerp(q.sx, q.sy, f[r][0].y, f[r][1].y, f[r][2].y, f[r][3].y);
+        o.z = bilerp(q.sx, q.sy, f[r][0].z, f[r][1].z, f[r][2].z, f[r][3].z);
+        o.w = bilerp(q.sx, q.sy, f[r][0].w, f[r][1].w, f[r][2].w, f[r][3].w);
+        reinterpret_cast<float4 *>(a.out)[j * a.iw + i] = o;
+    }
+}
+
 // Visits every cell of the rectangle [x0, x0+rw) x [y0, y0+rh) once across
 // the NT threads of a block, row-major, with an incremental (x, y) update
 // instead of a division per cell.

@@ -34,11 +34,12 @@ __global__ __launch_bounds__(kBX *kBY) void k_advect(const float *__restrict__ f
         o[base + k] = bilerp(t.sx, t.sy, f[a00 + k], f[a10 + k], f[a01 + k], f[a11 + k]);
 }
 
+constexpr int kColorRows = 4;  // dye pixels per thread, kBY rows apart
+
 __global__ __launch_bounds__(kBX *kBY) void k_advect_color(ColorArgs a) {
     const int i = blockIdx.x * kBX + threadIdx.x;
-    const int j = blockIdx.y * kBY + threadIdx.y;
-    if (i >= a.iw || j >= a.ih) return;
-    advect_color_pixel(a, i, j);
+    const int j = blockIdx.y * (kBY * kColorRows) + threadIdx.y;
+    advect_color_column<kColorRows>(a, i, j, kBY);
 }
 
 // One Jacobi smoothing sweep of channels 0,1 (src/fluid.cpp:154-185).
@@ -198,7 +199,8 @@ ColorArgs color_args(const float *img, float *out, const float *vp, int iw, int 
 int launch_advect_color(const float *img, float *out, const float *vp, int iw, int ih, int ic,
                         int vw, int vh, int vc, float dt, hipStream_t s) {
     const ColorArgs a = color_args(img, out, vp, iw, ih, ic, vw, vh, vc, dt);
-    hipLaunchKernelGGL(k_advect_color, grid_for(iw, ih), kBlock, 0, s, a);
+    const dim3 grid((iw + kBX - 1) / kBX, (ih + kBY * kColorRows - 1) / (kBY * kColorRows));
+    hipLaunchKernelGGL(k_advect_color, grid, kBlock, 0, s, a);
     return check_launch("advect_color");
 }
 
