@@ -175,6 +175,15 @@ def test_fluid_run_on_a_stream_and_per_op(orc, k):
     assert_bitexact(_run(s, (5, 4)), want)
 
 
+@pytest.mark.parametrize("k", [17, 30])
+def test_fluid_run_with_sweep_chunks(orc, k):
+    """K above 8 runs in chunks through the iterate planes: captured in the
+    run's graphs like any other launch, 21 steps (16-step graph, a 2-step
+    graph, eager steps) bit-exact."""
+    s = scenes.random_scene(130, 67, 130, 67, seed=48, dt=0.6, visc=0.2, iters=k, vel_scale=8.0)
+    assert_bitexact(_run(s, (21,)), oracle_run(orc, s, 21))
+
+
 def test_fluid_run_after_the_workspace_grows(orc):
     """A larger field on the same stream regrows the step workspace between
     two runs of a small one: the small field's next run must not replay a
