@@ -9,8 +9,10 @@ ctypes bindings for two CPU implementations of the reference's solver:
   where it was built; used to write and re-check the golden vectors and as
   the optional "reference" CPU baseline).
 
-Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline
-leg import this package.  The product library never touches it.
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py`` import this
+package: ``bench.py`` times it as the CPU baseline and, outside every timed
+region, uses it as the checker of its legs' results.  The product library
+never touches it.
 
 Fields are numpy arrays of shape (H, W, C), C-contiguous, matching
 ``vp_field``'s row-major interleaved ``[y][x][c]`` layout

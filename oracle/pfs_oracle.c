@@ -5,8 +5,9 @@
  * the buffer ping-pong, the stale channels and the even/odd Jacobi-count
  * behaviour of that file reproduced exactly.  It is the checker the HIP
  * library is compared against, bit for bit.  Only tests/,
- * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the
- * product library never links or calls it.
+ * __graft_entry__.smoke() and bench.py (the CPU baseline, and the checker of
+ * its legs' results outside the timed regions) may load it; the product
+ * library never links or calls it.
  *
  * Two instantiations:
  *   orc_*    float, the reference's own precision (parity oracle)

@@ -3,7 +3,8 @@
  *
  * TEST INFRASTRUCTURE ONLY.  This is a plain restatement of the reference's
  * CPU solver (src/fluid.cpp) used as the checker for the HIP library.  Only
- * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+ * tests/, __graft_entry__.smoke() and bench.py (the CPU baseline, and the
+ * checker of its legs' results) may load it.
  *
  * The including file defines:
  *   R        scalar type (float or double)
