@@ -70,6 +70,7 @@ class NeighborGrid:
         self.counts = torch.empty(n, **i32)
         self.offsets = torch.empty(n, **i32)
         self.total = torch.zeros(1, **i32)
+        self.nbr = None  # list buffer kept between calls of neighbors()
 
     @property
     def n(self) -> int:
@@ -104,15 +105,29 @@ class NeighborGrid:
 
     def neighbors(self, stream=None, xs=None, ys=None, by_position=False):
         """(counts, offsets, lists): particle i's neighbours are
-        lists[offsets[i] : offsets[i] + counts[i]].  Synchronises the stream
-        to size the lists."""
-        self.neighbor_counts(stream, xs, ys, by_position)
+        lists[offsets[i] : offsets[i] + counts[i]].  All three are the grid's
+        own buffers, overwritten by its next count or list call.  Synchronises
+        the stream to size the lists: one counting pass when the list buffer
+        kept from an earlier call holds them, else a sizing pass first (the
+        buffer then grows with 25% headroom)."""
+        def lists():
+            _lib.call("pfs_ngrid_neighbors", *self._common(xs, ys, by_position),
+                      _ptr(self.counts), _ptr(self.offsets), _ptr(self.nbr), self.nbr.numel(),
+                      _ptr(self.total), _ptr(self.ws), self.ws.numel() * 4, _stream(stream))
+            return self.counts, self.offsets, self.nbr[:int(self.total.item()) & 0xFFFFFFFF]
+
+        if self.nbr is not None:
+            try:
+                return lists()
+            except _lib.PfsError as e:  # more entries than the buffer: counted, not listed
+                if "capacity" not in str(e):
+                    raise
+        else:
+            self.neighbor_counts(stream, xs, ys, by_position)
         total = int(self.total.item()) & 0xFFFFFFFF
-        nbr = torch.empty(max(total, 1), dtype=torch.int32, device=self.keys.device)
-        _lib.call("pfs_ngrid_neighbors", *self._common(xs, ys, by_position), _ptr(self.counts),
-                  _ptr(self.offsets), _ptr(nbr), nbr.numel(), _ptr(self.total), _ptr(self.ws),
-                  self.ws.numel() * 4, _stream(stream))
-        return self.counts, self.offsets, nbr[:total]
+        self.nbr = torch.empty(max(total + total // 4, 1), dtype=torch.int32,
+                               device=self.keys.device)
+        return lists()
 
     def density(self, mass: float, stream=None, xs=None, ys=None, by_position=False,
                 out: torch.Tensor = None, with_counts: bool = False):
