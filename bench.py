@@ -500,6 +500,15 @@ def particles_leg(steps: int, warmup: int = 5) -> dict:
         lists = tr.grid.neighbors(None, xs0, ys0, by_position=True)
     torch.cuda.synchronize()
     lists_ms = (time.perf_counter() - t0) / 10 * 1e3
+    # The same lists from counts a density pass has just made (its counts are
+    # those of xs0, ys0): no counting pass.
+    tr.grid.density(1.0, None, xs0, ys0, by_position=True, with_counts=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        tr.grid.neighbors(None, xs0, ys0, by_position=True, counted=True)
+    torch.cuda.synchronize()
+    lists_counted_ms = (time.perf_counter() - t0) / 10 * 1e3
     lists = tuple(t.clone() for t in lists)
     torch.cuda.synchronize()
     e0.record()
@@ -522,7 +531,8 @@ def particles_leg(steps: int, warmup: int = 5) -> dict:
                  and lam.cpu().numpy().tobytes() == olam.tobytes()
                  and bool(np.allclose(res, ores, rtol=1e-12)))
     pbf = {"iterations": pbf_iters, "eps": eps, "rest_density": rho0,
-           "lists_ms": lists_ms, "ms_per_solve": pbf_ms, "ms_per_iteration": pbf_ms / pbf_iters,
+           "lists_ms": lists_ms, "lists_from_density_counts_ms": lists_counted_ms,
+           "ms_per_solve": pbf_ms, "ms_per_iteration": pbf_ms / pbf_iters,
            "rms_C_first_last": [float(np.sqrt(res[0] / n)), float(np.sqrt(res[-1] / n))],
            "bit_exact_vs_oracle": bool(pbf_exact),
            "checked": "positions and lambdas bit for bit, FP64 residuals to 1e-12"}

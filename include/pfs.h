@@ -302,6 +302,14 @@ int pfs_ngrid_neighbors(const pfs_ngrid *g, const float *xs, const float *ys, co
                         const uint32_t *cell_start, const uint32_t *cell_end, uint32_t *counts,
                         uint32_t *offsets, uint32_t *nbr, size_t nbr_capacity, uint32_t *total,
                         void *workspace, size_t workspace_bytes, void *stream);
+/* pfs_ngrid_neighbors' offsets, *total and lists from counts the caller
+ * already holds for the same positions, order and grid (those of
+ * pfs_ngrid_density with counts, say), without counting again.  Counts that
+ * are not the traversal's give wrong lists, never writes past nbr_capacity. */
+int pfs_ngrid_lists(const pfs_ngrid *g, const float *xs, const float *ys, const uint32_t *order,
+                    const uint32_t *cell_start, const uint32_t *cell_end, const uint32_t *counts,
+                    uint32_t *offsets, uint32_t *nbr, size_t nbr_capacity, uint32_t *total,
+                    void *workspace, size_t workspace_bytes, void *stream);
 /* rho[i] = (mass * 4/(pi r^8)) * S, S the FP32 sum over i and its neighbours
  * of (r^2 - d^2)^3 (2D poly6) in the traversal order (i where its cell's
  * range meets it); counts (may be NULL) gets the neighbour counts of the same

@@ -120,6 +120,8 @@ SIGNATURES = {
     "pfs_ngrid_build": [_G, _V, _V, _V, _V, _V, _V, _V, _V, _V, ctypes.c_size_t, _V],
     "pfs_ngrid_neighbors": [_G, _V, _V, _V, _V, _V, _V, _V, _V, ctypes.c_size_t, _V, _V,
                             ctypes.c_size_t, _V],
+    "pfs_ngrid_lists": [_G, _V, _V, _V, _V, _V, _V, _V, _V, ctypes.c_size_t, _V, _V,
+                        ctypes.c_size_t, _V],
     "pfs_ngrid_density": [_G, _V, _V, _V, _V, _V, _F, _V, _V, _V, ctypes.c_size_t, _V],
     "pfs_scan_exclusive_u32": [_V, _V, _I, _V, _V, ctypes.c_size_t, _V],
     "pfs_gather_u32": [_V, _V, _V, _I, _V],

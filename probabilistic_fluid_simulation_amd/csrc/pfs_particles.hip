@@ -942,6 +942,28 @@ PbfWs pbf_ws(void *ws, int n, int max_iters) {
     return w;
 }
 
+// Offsets and *total from counts, then the lists (checked against the
+// capacity first, which waits for *total).
+int lists_from_counts(const pfs_ngrid *g, const NArgs &a, const uint32_t *counts,
+                      uint32_t *offsets, uint32_t *nbr, size_t nbr_capacity, uint32_t *total,
+                      const SortWs &w, hipStream_t s) {
+    PFS_TRY(scan_exclusive(counts, offsets, g->n, total, w.sums, s));
+    uint32_t h_total = 0;
+    PFS_HIP_TRY(hipMemcpyAsync(&h_total, total, sizeof h_total, hipMemcpyDeviceToHost, s));
+    PFS_HIP_TRY(hipStreamSynchronize(s));
+    if (h_total > nbr_capacity)
+        return set_error(PFS_ERR_INVALID, "ngrid: neighbour list capacity " +
+                                              std::to_string(nbr_capacity) + " < " +
+                                              std::to_string(h_total));
+    if (a.order)
+        hipLaunchKernelGGL(k_nbr_fill, dim3(grid_for(g->n)), dim3(256), 0, s, a, offsets, nbr,
+                           nbr_capacity);
+    else
+        hipLaunchKernelGGL(k_nbr_fill_lds, dim3(grid_for(g->n)), dim3(256), 0, s, a, offsets,
+                           counts, nbr, nbr_capacity);
+    return check_launch("neighbour lists");
+}
+
 }  // namespace
 }  // namespace pfs
 
@@ -1081,23 +1103,26 @@ int pfs_ngrid_neighbors(const pfs_ngrid *g, const float *xs, const float *ys, co
     SortWs w = sort_ws(ws, g->n);
     const NArgs a = nargs(g, xs, ys, order, w.k0, cell_start, cell_end);
     hipLaunchKernelGGL(k_nbr_count, dim3(grid_for(g->n)), dim3(256), 0, s, a, counts);
-    PFS_TRY(scan_exclusive(counts, offsets, g->n, total, w.sums, s));
-    if (!nbr) return PFS_OK;  // counts and offsets only
-    uint32_t h_total = 0;
-    PFS_HIP_TRY(hipMemcpyAsync(&h_total, total, sizeof h_total, hipMemcpyDeviceToHost, s));
-    PFS_HIP_TRY(hipStreamSynchronize(s));
-    if (h_total > nbr_capacity)
-        return set_error(PFS_ERR_INVALID, "ngrid: neighbour list capacity " +
-                                              std::to_string(nbr_capacity) + " < " +
-                                              std::to_string(h_total));
-    if (order)
-        hipLaunchKernelGGL(k_nbr_fill, dim3(grid_for(g->n)), dim3(256), 0, s, a, offsets, nbr,
-                           nbr_capacity);
-    else
-        hipLaunchKernelGGL(k_nbr_fill_lds, dim3(grid_for(g->n)), dim3(256), 0, s, a, offsets,
-                           counts, nbr, nbr_capacity);
-    return check_launch("neighbour lists");
+    if (!nbr) return scan_exclusive(counts, offsets, g->n, total, w.sums, s);  // no lists
+    return lists_from_counts(g, a, counts, offsets, nbr, nbr_capacity, total, w, s);
 }
+
+int pfs_ngrid_lists(const pfs_ngrid *g, const float *xs, const float *ys, const uint32_t *order,
+                    const uint32_t *cell_start, const uint32_t *cell_end, const uint32_t *counts,
+                    uint32_t *offsets, uint32_t *nbr, size_t nbr_capacity, uint32_t *total,
+                    void *ws, size_t ws_bytes, void *stream) {
+    if (!valid_grid(g)) return set_error(PFS_ERR_INVALID, "ngrid: not initialised by pfs_ngrid_init");
+    if (!xs || !ys || !cell_start || !cell_end || !counts || !offsets || !nbr || !total)
+        return set_error(PFS_ERR_INVALID, "ngrid: NULL buffer");
+    if (!ws || ws_bytes < pfs_ngrid_workspace_bytes(g) || ((uintptr_t)ws & 3))
+        return set_error(PFS_ERR_INVALID,
+                         "ngrid: workspace not 4-byte aligned or smaller than pfs_ngrid_workspace_bytes()");
+    SortWs w = sort_ws(ws, g->n);
+    const NArgs a = nargs(g, xs, ys, order, w.k0, cell_start, cell_end);
+    return lists_from_counts(g, a, counts, offsets, nbr, nbr_capacity, total, w,
+                             (hipStream_t)stream);
+}
+
 
 size_t pfs_pbf_workspace_bytes(const pfs_ngrid *g, int max_iters) {
     if (!g || g->n < 1 || max_iters < 0 || max_iters > PFS_PBF_MAX_ITERS) return 0;

@@ -103,19 +103,25 @@ class NeighborGrid:
                   self.ws.numel() * 4, _stream(stream))
         return self.counts, self.offsets
 
-    def neighbors(self, stream=None, xs=None, ys=None, by_position=False):
+    def neighbors(self, stream=None, xs=None, ys=None, by_position=False, counted=False):
         """(counts, offsets, lists): particle i's neighbours are
         lists[offsets[i] : offsets[i] + counts[i]].  All three are the grid's
         own buffers, overwritten by its next count or list call.  Synchronises
         the stream to size the lists: one counting pass when the list buffer
         kept from an earlier call holds them, else a sizing pass first (the
-        buffer then grows with 25% headroom)."""
+        buffer then grows with 25% headroom).  counted=True: the grid's counts
+        already hold those of these positions (density(..., with_counts=True)
+        just ran on them), so no counting pass runs (pfs_ngrid_lists)."""
+        fn = "pfs_ngrid_lists" if counted else "pfs_ngrid_neighbors"
+
         def lists():
-            _lib.call("pfs_ngrid_neighbors", *self._common(xs, ys, by_position),
-                      _ptr(self.counts), _ptr(self.offsets), _ptr(self.nbr), self.nbr.numel(),
-                      _ptr(self.total), _ptr(self.ws), self.ws.numel() * 4, _stream(stream))
+            _lib.call(fn, *self._common(xs, ys, by_position), _ptr(self.counts),
+                      _ptr(self.offsets), _ptr(self.nbr), self.nbr.numel(), _ptr(self.total),
+                      _ptr(self.ws), self.ws.numel() * 4, _stream(stream))
             return self.counts, self.offsets, self.nbr[:int(self.total.item()) & 0xFFFFFFFF]
 
+        if self.nbr is None and counted:  # the first call sizes the buffer from its total
+            self.nbr = torch.empty(1, dtype=torch.int32, device=self.keys.device)
         if self.nbr is not None:
             try:
                 return lists()
@@ -217,8 +223,8 @@ class Tracers:
     def neighbor_counts(self, stream=None):
         return self.grid.neighbor_counts(stream, self.x, self.y, by_position=True)
 
-    def neighbors(self, stream=None):
-        return self.grid.neighbors(stream, self.x, self.y, by_position=True)
+    def neighbors(self, stream=None, counted=False):
+        return self.grid.neighbors(stream, self.x, self.y, by_position=True, counted=counted)
 
     def density(self, mass: float, stream=None, out=None, with_counts: bool = False):
         return self.grid.density(mass, stream, self.x, self.y, by_position=True, out=out,

@@ -241,6 +241,31 @@ def test_pbf_small_counts(n):
     np.testing.assert_allclose(res.cpu().numpy(), ores, rtol=1e-12)
 
 
+@pytest.mark.parametrize("order", [True, False], ids=["by-id", "by-position"])
+def test_lists_from_density_counts(order):
+    """pfs_ngrid_lists: lists from the density pass's counts equal those of
+    pfs_ngrid_neighbors, for a fresh grid (buffer sized on the first call),
+    a reused buffer, and one that must grow."""
+    n, W = 6000, 40.0
+    rng = np.random.default_rng(29)
+    x = (W / 2 + rng.normal(0, 6.0, n)).astype(np.float32) % np.float32(W)
+    y = (W / 2 + rng.normal(0, 6.0, n)).astype(np.float32) % np.float32(W)
+    ng = particles.NeighborGrid(n, W, W, 1.0)
+    ng.build(torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda())
+    og = op.Grid(x, y, W, W, 1.0, ng.g.gx, ng.g.gy, ng.g.kx, ng.g.ky)
+    oc, oo, on = og.neighbors(by_position=not order)
+    for _ in range(2):
+        ng.density(1.0, by_position=not order, with_counts=True)
+        counts, offsets, nbr = ng.neighbors(by_position=not order, counted=True)
+        np.testing.assert_array_equal(_u32(counts), oc)
+        np.testing.assert_array_equal(_u32(offsets), oo)
+        np.testing.assert_array_equal(_u32(nbr), on)
+    ng.nbr = ng.nbr[:10].clone()  # too small: grows
+    ng.density(1.0, by_position=not order, with_counts=True)
+    np.testing.assert_array_equal(_u32(ng.neighbors(by_position=not order, counted=True)[2]), on)
+    assert ng.nbr.numel() >= len(on)
+
+
 @pytest.mark.parametrize("spread", [2.5, 4.0], ids=["dense", "sparse"])
 def test_lists_by_position_bitexact(spread):
     """Lists by sorted position (written through LDS windows): a dense blob
