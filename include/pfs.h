@@ -77,6 +77,16 @@ int pfs_diffuse(pfs_field *vp, pfs_field *vp_out, float viscosity, float dt, int
 /* includes/fluid.hpp:70 / src/fluid.cpp:198-208: a no-op in the reference. */
 int pfs_add_forces(pfs_field *vp, const float *forces, void *stream);
 
+/* Force injection, which the reference declares (includes/fluid.hpp:61-71:
+ * "forces - A 3D array of force values for each pixel in the grid") but
+ * leaves as an empty loop over every cell and channel (src/fluid.cpp:198-208)
+ * whose call in the step is commented out (src/fluid.cpp:302, between
+ * diffuse and computePressure).  Opt-in here:
+ *   vp[j][i][k] += dt * forces[j][i][k]   for k < fz (fz <= vp->z),
+ * forces a device array [vp->y][vp->x][fz]; the product is rounded, then the
+ * sum (no fused multiply-add).  fz = 2 pushes the velocity only. */
+int pfs_add_forces_apply(pfs_field *vp, const float *forces, int fz, float dt, void *stream);
+
 /* includes/fluid.hpp:81 / src/fluid.cpp:210-267.  Divergence into channel 3
  * of both fields, then `iters` Jacobi sweeps of channel 2 with the same
  * pointer swaps as pfs_diffuse.  Needs z >= 4. */

@@ -109,6 +109,16 @@ class Oracle:
                             self.real(dt), ctypes.c_int(iters))
         return self.b.back(f, (vp, out))
 
+    def add_forces_apply(self, vp, forces, dt):
+        """vp[..., :fz] += dt * forces (forces (H, W, fz)), in place."""
+        f = self.b.make(vp)
+        forces = np.ascontiguousarray(forces, dtype=self.b.dtype)
+        if forces.shape[:2] != vp.shape[:2] or forces.ndim != 3:
+            raise ValueError("forces must be (H, W, fz)")
+        self._fn("add_forces_apply")(ctypes.byref(f[0]),
+                                     forces.ctypes.data_as(ctypes.POINTER(self.b.ctype)),
+                                     ctypes.c_int(forces.shape[2]), self.real(dt))
+
     def compute_pressure(self, vp, out, dt, iters) -> Tuple[np.ndarray, np.ndarray]:
         f = self.b.make(vp, out)
         self._fn("compute_pressure")(ctypes.byref(f[0]), ctypes.byref(f[1]), self.real(dt),

@@ -141,6 +141,24 @@ void NAME(add_forces)(FIELD *vp, R *forces) {
 }
 
 /*
+ * Force injection, filling in the reference's empty loop over every cell
+ * and channel (src/fluid.cpp:198-208; "forces - A 3D array of force values
+ * for each pixel in the grid", includes/fluid.hpp:61-66) with the explicit
+ * Euler update u += dt * f of the first fz channels, forces laid out
+ * [y][x][fz].  No reference behaviour to match: the library's
+ * pfs_add_forces_apply is checked against this.
+ */
+void NAME(add_forces_apply)(FIELD *vp, const R *forces, int fz, R dt) {
+    const int w = vp->x, h = vp->y, c = vp->z;
+    for (int x = 0; x < w; x++)
+        for (int y = 0; y < h; y++)
+            for (int z = 0; z < fz; z++) {
+                const R f = dt * forces[(y * w + x) * fz + z];
+                vp->data[ORC_IDX(x, y, z, w, c)] = vp->data[ORC_IDX(x, y, z, w, c)] + f;
+            }
+}
+
+/*
  * Divergence into channel 3 of both buffers, then K Jacobi sweeps of the
  * pressure Poisson equation on channel 2 (src/fluid.cpp:210-267).
  */

@@ -66,6 +66,7 @@ SIGNATURES = {
     "pfs_advect_color": [_P, _P, _P, _F, _V],
     "pfs_diffuse": [_P, _P, _F, _F, _I, _V],
     "pfs_add_forces": [_P, _V, _V],
+    "pfs_add_forces_apply": [_P, _V, _I, _F, _V],
     "pfs_compute_pressure": [_P, _P, _F, _I, _V],
     "pfs_subtract_pressure_gradient": [_P, _P, _F, _V],
     "pfs_simulate_fluid_step": [_P, _P, _F, _F, _I, _V],

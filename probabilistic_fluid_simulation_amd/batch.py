@@ -44,9 +44,13 @@ class Batch:
             pass
 
     def add(self, vp: VPField, tmp: VPField, image: VPField, itmp: VPField, dt: float,
-            viscosity: float, iters: int) -> int:
-        """Add a scene (its buffers stay caller-owned); returns its slot."""
-        torch.cuda.current_stream().synchronize()  # inputs written on torch's stream
+            viscosity: float, iters: int, sync: bool = True) -> int:
+        """Add a scene (its buffers stay caller-owned); returns its slot.
+        sync: first wait for torch's current stream, where the inputs were
+        written; a caller that orders the batch's stream after its inputs
+        itself (an event) passes False."""
+        if sync:
+            torch.cuda.current_stream().synchronize()
         slot = ctypes.c_int()
         structs = [f._struct() for f in (vp, tmp, image, itmp)]
         _lib.call("pfs_batch_add", self._h, *[ctypes.byref(s) for s in structs], dt, viscosity,
@@ -54,11 +58,16 @@ class Batch:
         self._scenes[slot.value] = (vp, tmp, image, itmp)
         return slot.value
 
-    def remove(self, slot: int) -> None:
+    def remove(self, slot: int, hold: bool = True) -> tuple:
         """Remove a scene; later steps no longer touch it.  Its buffers must
-        outlive the steps already queued, so they are held until sync()."""
+        outlive the steps already queued: with hold, the batch keeps them
+        until sync(); without, the caller does (they are returned either
+        way)."""
         _lib.call("pfs_batch_remove", self._h, slot)
-        self._retired.append(self._scenes.pop(slot))
+        bufs = self._scenes.pop(slot)
+        if hold:
+            self._retired.append(bufs)
+        return bufs
 
     def step(self, n: int = 1) -> None:
         _lib.call("pfs_batch_step", self._h, n)

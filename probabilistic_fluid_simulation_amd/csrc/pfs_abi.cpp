@@ -16,6 +16,7 @@
 
 #include "pfs.h"
 #include "pfs_internal.h"
+#include "pfs_step_kernels.h"
 
 namespace pfs {
 
@@ -195,6 +196,14 @@ int pfs_add_forces(pfs_field *vp, const float *forces, void *stream) {
     return PFS_OK;  // src/fluid.cpp:198-208 is an empty loop
 }
 
+int pfs_add_forces_apply(pfs_field *vp, const float *forces, int fz, float dt, void *stream) {
+    PFS_TRY(check_field(vp, "vp", 1));
+    if (!forces) return set_error(PFS_ERR_INVALID, "forces is NULL");
+    if (fz < 1 || fz > vp->z) return set_error(PFS_ERR_INVALID, "need 1 <= fz <= vp->z");
+    if (forces == vp->data) return set_error(PFS_ERR_INVALID, "forces and vp alias");
+    return launch_add_forces(vp->data, forces, vp->x, vp->y, vp->z, fz, dt, (hipStream_t)stream);
+}
+
 int pfs_compute_pressure(pfs_field *vp, pfs_field *vp_out, float dt, int iters, void *stream) {
     PFS_TRY(check_field(vp, "vp", 4));
     PFS_TRY(check_field(vp_out, "vp_out", 4));
@@ -241,10 +250,15 @@ static int check_step_fields(const pfs_field *vp, const pfs_field *tmp, int iter
     return PFS_OK;
 }
 
-// Whether the fused kernels apply: 4-channel, 16-byte aligned fields, K >= 1.
+// Sweeps one fused step plans for: 64 launches of kMaxChunk sweeps.
+constexpr int kFusedMaxIters = 64 * kMaxChunk;
+
+// Whether the fused kernels apply: 4-channel, 16-byte aligned fields,
+// 1 <= K <= kFusedMaxIters (a larger K takes the per-operator sequence,
+// which accepts any count, as the reference does).
 static bool fused_ok(const pfs_field *vp, const pfs_field *tmp, int iters) {
-    return g_step_path == PFS_PATH_AUTO && vp->z == 4 && iters >= 1 && aligned16(vp->data) &&
-           aligned16(tmp->data);
+    return g_step_path == PFS_PATH_AUTO && vp->z == 4 && iters >= 1 && iters <= kFusedMaxIters &&
+           aligned16(vp->data) && aligned16(tmp->data);
 }
 
 // simulate_fluid_step, optionally followed by advect_color_step on

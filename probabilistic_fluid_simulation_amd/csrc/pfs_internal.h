@@ -68,6 +68,8 @@ int launch_divergence(float *in, float *out, int w, int h, int c, float gamma, h
 int launch_pressure_sweep(const float *in, float *out, int w, int h, int c, hipStream_t s);
 int launch_subtract_gradient(const float *in, float *out, int w, int h, int c, float dt,
                              hipStream_t s);
+int launch_add_forces(float *vp, const float *f, int w, int h, int c, int fz, float dt,
+                      hipStream_t s);
 int launch_advect_keys(const float *f, int *keys, int w, int h, int c, float dt, hipStream_t s);
 int launch_advect_color_keys(const float *vp, int *keys, int iw, int ih, int vw, int vh, int vc,
                              float dt, hipStream_t s);

@@ -153,6 +153,20 @@ __global__ __launch_bounds__(kBX *kBY) void k_advect_color_keys(
     k[5] = t.j1;
 }
 
+// Force injection (the step the reference leaves as a stub,
+// src/fluid.cpp:198-208, called nowhere: src/fluid.cpp:302 comments it
+// out): vp[cell][k] += dt * forces[cell][k] for channels k < fz, the
+// product rounded, then the sum (no fused multiply-add).
+__global__ __launch_bounds__(kBX *kBY) void k_add_forces(float *__restrict__ vp,
+                                                         const float *__restrict__ f, int w, int h,
+                                                         int c, int fz, float dt) {
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    if (i >= w || j >= h) return;
+    const int cell = j * w + i;
+    for (int k = 0; k < fz; ++k) vp[cell * c + k] = vp[cell * c + k] + dt * f[cell * fz + k];
+}
+
 namespace {
 dim3 grid_for(int w, int h) { return dim3((w + kBX - 1) / kBX, (h + kBY - 1) / kBY); }
 const dim3 kBlock(kBX, kBY);
@@ -225,6 +239,12 @@ int launch_subtract_gradient(const float *in, float *out, int w, int h, int c, f
                              hipStream_t s) {
     hipLaunchKernelGGL(k_subtract_gradient, grid_for(w, h), kBlock, 0, s, in, out, w, h, c, dt);
     return check_launch("subtract_gradient");
+}
+
+int launch_add_forces(float *vp, const float *f, int w, int h, int c, int fz, float dt,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_add_forces, grid_for(w, h), kBlock, 0, s, vp, f, w, h, c, fz, dt);
+    return check_launch("add_forces");
 }
 
 int launch_advect_keys(const float *f, int *keys, int w, int h, int c, float dt, hipStream_t s) {

@@ -557,7 +557,9 @@ __global__ void k_density(NArgs a, float mc, float *__restrict__ rho,
     const uint32_t i = a.id((uint32_t)p);
     if (i < (uint32_t)a.n) {
         rho[i] = mc * sum;
-        if (counts) counts[i] = cnt - 1u;  // p itself (d2 = 0) was counted
+        // A finite p met itself (d2 = 0) and counted it; a non-finite p
+        // counted nothing (every d2 it forms is NaN or Inf), as k_nbr_count.
+        if (counts) counts[i] = cnt ? cnt - 1u : 0u;
     }
 }
 

@@ -117,6 +117,19 @@ def addForces(vp: VPField, forces=None, stream=None) -> None:  # noqa: N802 (ref
     _lib.call("pfs_add_forces", b.ptr(0), None, _stream(stream))
 
 
+def add_forces_apply(vp: VPField, forces: torch.Tensor, dt: float, stream=None) -> None:
+    """Force injection the reference leaves as a stub (src/fluid.cpp:198-208,
+    its call commented out at :302): vp[..., :fz] += dt * forces, forces an
+    (H, W, fz) float32 device tensor, fz <= channels."""
+    if forces.dtype != torch.float32 or forces.dim() != 3 or not forces.is_contiguous():
+        raise ValueError("forces must be a contiguous (H, W, fz) float32 tensor")
+    if tuple(forces.shape[:2]) != (vp.y, vp.x):
+        raise ValueError("forces must have the field's height and width")
+    b = _Bound(vp)
+    _lib.call("pfs_add_forces_apply", b.ptr(0), ctypes.c_void_p(forces.data_ptr()),
+              int(forces.shape[2]), dt, _stream(stream))
+
+
 def computePressure(vp: VPField, vp_out: VPField, dt: float, iters: int = NUM_JACOBI_ITERS,
                     stream=None) -> None:  # noqa: N802
     """includes/fluid.hpp:81, src/fluid.cpp:210-267."""

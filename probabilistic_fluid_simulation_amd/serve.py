@@ -3,8 +3,9 @@
 The reference runs one simulation per process (src/main.cpp).  Here a
 request loop admits jobs as they arrive, keeps every admitted job's state
 resident on the device, and advances all of them together through a
-``Batch`` (pfs_batch_*): after every batched substep, finished jobs are
-retired (their dye copied out) and waiting jobs are admitted as slots and
+``Batch`` (pfs_batch_*): runs of batched steps are queued ahead of the
+device, jobs that finish in a run leave the batch behind it (their dye
+copied out on the same stream), and waiting jobs are admitted as slots and
 the HBM budget allow.  Each job's result is bit-identical to running it
 alone: scenes never interact inside the batched kernels.
 
@@ -14,6 +15,7 @@ path does not wait on host numpy.
 """
 from __future__ import annotations
 
+import collections
 import dataclasses
 import math
 import time
@@ -44,6 +46,7 @@ class Job:
     admitted: Optional[float] = None
     finished: Optional[float] = None
     result: Optional[torch.Tensor] = None
+    rejected: bool = False  # larger than the server's whole HBM budget
 
     @property
     def nbytes(self) -> int:
@@ -103,64 +106,135 @@ def jobs_from_specs(specs: List[dict], iters: int = 4) -> List[Job]:
                 ih=s["ih"], steps=s["steps"], seed=s["seed"], iters=iters) for s in specs]
 
 
+class _Device:
+    """The GPU side of a Server: the batch's stream, a side stream on which
+    admitted jobs' inputs are generated while the batch runs, and events."""
+
+    def __init__(self, batch: Batch):
+        self.stream = torch.cuda.ExternalStream(batch.stream_handle)
+        self.side = torch.cuda.Stream()
+
+    def make(self, job: "Job"):
+        """The job's buffers, generated on the side stream; the batch's
+        stream waits for them before its next step."""
+        with torch.cuda.stream(self.side):
+            bufs = device_scene(job)
+        ready = torch.cuda.Event()
+        ready.record(self.side)
+        self.stream.wait_event(ready)
+        return bufs
+
+    def mark(self):
+        """An event after everything queued so far on the batch's stream."""
+        ev = torch.cuda.Event()
+        ev.record(self.stream)
+        return ev
+
+    def copy_out(self, t: torch.Tensor) -> torch.Tensor:
+        """A copy of t made on the batch's stream after the work queued there."""
+        with torch.cuda.stream(self.stream):
+            return t.clone()
+
+
 class Server:
     """Admission + continuous batching over one Batch.
 
-    hbm_budget: bytes the resident jobs may use; max_active: slot limit;
-    max_substeps: longest run of steps between admission checks."""
+    hbm_budget: bytes the resident jobs may use (a job larger than the whole
+    budget is rejected, never queued); max_active: slot limit (>= 1);
+    max_substeps: longest run of steps between admission points; depth:
+    runs of steps queued on the device ahead of the host (>= 1), so that the
+    device never waits for the host to retire and admit jobs.
+
+    Nothing synchronises the device with the host per step: a run of n
+    steps is queued, jobs that finish in it are removed from later steps and
+    their results copied on the batch's stream right behind it, and an event
+    marks its end.  The host then queues the next run (up to `depth` ahead)
+    and meanwhile admits arrivals, whose inputs are generated on a side
+    stream.  A job counts as finished, and its memory as free, when the host
+    sees its run's event complete.
+
+    `batch` and `device` replace the Batch and its GPU side (tests drive the
+    admission logic with stand-ins on the CPU)."""
 
     def __init__(self, hbm_budget: float = 200e9, max_active: int = 256, max_substeps: int = 16,
-                 keep_results: bool = False):
-        self.batch = Batch()
+                 keep_results: bool = False, depth: int = 2, batch=None, device=None):
+        if not hbm_budget > 0:
+            raise ValueError("hbm_budget must be positive")
+        if max_active < 1 or max_substeps < 1 or depth < 1:
+            raise ValueError("max_active, max_substeps and depth must be >= 1")
+        self.batch = batch if batch is not None else Batch()
+        self.dev = device if device is not None else _Device(self.batch)
         self.hbm_budget = hbm_budget
         self.max_active = max_active
         self.max_substeps = max_substeps
         self.keep_results = keep_results
-        self.active: Dict[int, tuple] = {}  # slot -> (job, remaining, buffers)
-        self.used = 0
+        self.depth = depth
+        self.active: Dict[int, tuple] = {}  # slot -> (job, steps not yet queued, buffers)
+        self.used = 0  # bytes of admitted jobs whose last run has not been seen to end
+        self.peak_active = 0
 
     def _admit(self, job: Job, now: float) -> None:
-        bufs = device_scene(job)
-        slot = self.batch.add(*bufs, job.dt, job.visc, job.iters)
+        bufs = self.dev.make(job)
+        slot = self.batch.add(*bufs, job.dt, job.visc, job.iters, sync=False)
         job.admitted = now
         self.active[slot] = (job, job.steps, bufs)
         self.used += job.nbytes
+        self.peak_active = max(self.peak_active, len(self.active))
 
-    def _retire(self, slot: int, now: float) -> None:
-        job, _, bufs = self.active.pop(slot)
-        if self.keep_results:
-            job.result = self.batch.image(slot).clone()
-        self.batch.remove(slot)
-        job.finished = now
-        self.used -= job.nbytes
+    def _queue_run(self) -> tuple:
+        """Queues the steps up to the next completion (at most max_substeps)
+        for every resident job; returns (end event, finishing jobs)."""
+        n = min(min(rem for _, rem, _ in self.active.values()), self.max_substeps)
+        self.batch.step(n)
+        done = []
+        for slot in list(self.active):
+            job, rem, bufs = self.active[slot]
+            if rem == n:
+                res = self.dev.copy_out(self.batch.image(slot)) if self.keep_results else None
+                bufs = self.batch.remove(slot, hold=False)
+                del self.active[slot]
+                done.append((job, bufs, res))
+            else:
+                self.active[slot] = (job, rem - n, bufs)
+        return self.dev.mark(), done
 
-    def run(self, jobs: List[Job], clock=time.perf_counter) -> dict:
-        pending = sorted(jobs, key=lambda j: j.arrival)
-        i = 0
+    def run(self, jobs: List[Job], clock=time.perf_counter, sleep=time.sleep) -> dict:
+        for j in jobs:
+            j.rejected = j.nbytes > self.hbm_budget
+        pending = collections.deque(sorted((j for j in jobs if not j.rejected),
+                                           key=lambda j: j.arrival))
+        inflight: collections.deque = collections.deque()
         t0 = clock()
-        while i < len(pending) or self.active:
+        while pending or self.active or inflight:
             now = clock() - t0
-            while (i < len(pending) and pending[i].arrival <= now
+            while (pending and pending[0].arrival <= now
                    and len(self.active) < self.max_active
-                   and self.used + pending[i].nbytes <= self.hbm_budget):
-                self._admit(pending[i], now)
-                i += 1
-            if not self.active:
-                if i < len(pending):
-                    time.sleep(max(0.0, pending[i].arrival - (clock() - t0)))
+                   and self.used + pending[0].nbytes <= self.hbm_budget):
+                self._admit(pending.popleft(), now)
+            if self.active and len(inflight) < self.depth:
+                inflight.append(self._queue_run())
                 continue
-            # Advance every resident job to the next completion, at most
-            # max_substeps steps, so that arrivals wait at most that long.
-            n = min(min(rem for _, rem, _ in self.active.values()), self.max_substeps)
-            self.batch.step(n)
-            self.batch.sync()
-            now = clock() - t0
-            for slot in list(self.active):
-                job, rem, bufs = self.active[slot]
-                if rem - n == 0:
-                    self._retire(slot, now)
-                else:
-                    self.active[slot] = (job, rem - n, bufs)
+            if inflight:
+                ev, done = inflight[0]
+                if not ev.query():
+                    nxt = pending[0] if pending else None
+                    if (nxt is not None and len(self.active) < self.max_active
+                            and self.used + nxt.nbytes <= self.hbm_budget):
+                        # The next job fits: admit it on arrival, while the
+                        # device works.
+                        sleep(min(max(0.0, nxt.arrival - (clock() - t0)), 1e-4))
+                    else:
+                        ev.synchronize()  # only the run's end can free room
+                    continue
+                inflight.popleft()
+                now = clock() - t0
+                for job, _bufs, res in done:
+                    job.finished = now
+                    job.result = res
+                    self.used -= job.nbytes
+                continue
+            # Idle: nothing resident, nothing queued; the next job is due later.
+            sleep(max(0.0, pending[0].arrival - (clock() - t0)))
         self.batch.sync()
         return summarize(jobs, clock() - t0)
 
@@ -171,7 +245,8 @@ class Server:
 def summarize(jobs: List[Job], makespan: float) -> dict:
     lat = np.array([j.latency for j in jobs if j.latency is not None])
     cells = sum(j.fw * j.fh * j.steps for j in jobs)
-    return {"jobs": len(jobs), "completed": int(lat.size), "makespan_s": makespan,
+    return {"jobs": len(jobs), "completed": int(lat.size),
+            "rejected": sum(1 for j in jobs if j.rejected), "makespan_s": makespan,
             "jobs_per_s": lat.size / makespan if makespan > 0 else 0.0,
             "p50_latency_s": float(np.percentile(lat, 50)) if lat.size else None,
             "p99_latency_s": float(np.percentile(lat, 99)) if lat.size else None,

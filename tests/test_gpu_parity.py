@@ -16,6 +16,7 @@ torch = pytest.importorskip("torch")
 from probabilistic_fluid_simulation_amd import fluid, scenes  # noqa: E402
 from probabilistic_fluid_simulation_amd.fluid import VPField  # noqa: E402
 
+from keys_util import color_keys_numpy  # noqa: E402
 from gpu_util import assert_bitexact, gpu_run, oracle_run, to_dev  # noqa: E402
 
 pytestmark = pytest.mark.gpu
@@ -393,3 +394,74 @@ def test_advect_keys_bitexact(shape):
     vp = VPField.from_numpy(s.vp)
     got = fluid.advect_keys(vp, s.dt).cpu().numpy()
     assert np.array_equal(got, _keys_numpy(s.vp, s.dt))
+
+
+@pytest.mark.parametrize("case", ["17x13_dye23x19", "256x192_dye1000x700", "1024x1024"])
+def test_advect_color_keys_bitexact(case):
+    """pfs_debug_advect_color_keys: the nearest-cell and corner keys of every
+    dye pixel equal the numpy restatement of src/fluid.cpp:82-113."""
+    if case == "1024x1024":
+        s = scenes.plain_1m()
+    elif case == "17x13_dye23x19":
+        s = scenes.random_scene(17, 13, 23, 19, seed=21, dt=0.8, vel_scale=9.0 * 23)
+    else:
+        s = scenes.random_scene(256, 192, 1000, 700, seed=22, dt=0.7, vel_scale=5.0 * 1000)
+    got = fluid.advect_color_keys(VPField.from_numpy(s.image), VPField.from_numpy(s.vp),
+                                  s.dt).cpu().numpy()
+    want = color_keys_numpy(s.image.shape[:2], s.vp, s.dt)
+    assert got.shape == want.shape
+    assert np.array_equal(got, want), int(np.count_nonzero(got != want))
+
+
+# ------------------------------------------------------------- force injection
+@pytest.mark.parametrize("fz,channels", [(1, 4), (2, 4), (4, 4), (2, 3)])
+def test_add_forces_apply_bitexact(orc, fz, channels):
+    """pfs_add_forces_apply against the oracle: vp[..., :fz] += dt * forces."""
+    rng = np.random.default_rng(40 + fz)
+    vp = rng.standard_normal((37, 70, channels)).astype(np.float32)
+    f = rng.standard_normal((37, 70, fz)).astype(np.float32)
+    dev = VPField.from_numpy(vp)
+    fluid.add_forces_apply(dev, torch.from_numpy(f).cuda(), 0.37)
+    want = vp.copy()
+    orc.add_forces_apply(want, f, 0.37)
+    assert dev.numpy().tobytes() == want.tobytes()
+
+
+def test_step_with_forces_at_the_reference_position(orc):
+    """The reference's step with its commented-out addForces call enabled
+    (src/fluid.cpp:298-305: advect, diffuse, addForces, computePressure,
+    subtractPressureGradient), through the per-operator entry points, equals
+    the oracle's same sequence bit for bit, buffers and swaps included."""
+    s = scenes.random_scene(40, 27, 4, 4, seed=44, dt=0.7, visc=0.2, iters=5)
+    f = np.random.default_rng(44).standard_normal((27, 40, 2)).astype(np.float32)
+    vp, tmp = VPField.from_numpy(s.vp), VPField.from_numpy(s.tmp)
+    fluid.advect(vp, tmp, s.dt)
+    fluid.diffuse(tmp, vp, s.visc, s.dt, s.iters)
+    fluid.add_forces_apply(vp, torch.from_numpy(f).cuda(), s.dt)
+    fluid.computePressure(vp, tmp, s.dt, s.iters)
+    fluid.subtractPressureGradient(tmp, vp, s.dt)
+    c = s.copy()
+    orc.advect(c.vp, c.tmp, c.dt)
+    t, v = orc.diffuse(c.tmp, c.vp, c.visc, c.dt, c.iters)
+    orc.add_forces_apply(v, f, c.dt)
+    v, t = orc.compute_pressure(v, t, c.dt, c.iters)
+    orc.subtract_pressure_gradient(t, v, c.dt)
+    assert vp.numpy().tobytes() == v.tobytes() and tmp.numpy().tobytes() == t.tobytes()
+
+
+def test_add_forces_apply_rejects_bad_arguments():
+    vp = VPField.from_numpy(np.zeros((4, 5, 4), np.float32))
+    with pytest.raises(ValueError):
+        fluid.add_forces_apply(vp, torch.zeros((4, 6, 2), device="cuda"), 0.1)
+    with pytest.raises(Exception):
+        fluid.add_forces_apply(vp, torch.zeros((4, 5, 5), device="cuda"), 0.1)  # fz > channels
+
+
+@pytest.mark.parametrize("iters", [512, 513])
+def test_iters_past_the_fused_plan(orc, iters):
+    """The reference accepts any NUM_JACOBI_ITERS.  512 sweeps are the most
+    one fused step plans (64 launches of 8); 513 takes the per-operator
+    sequence.  Both equal the oracle, through the step and the timestep."""
+    s = scenes.random_scene(24, 17, 24, 17, seed=50, dt=0.5, visc=0.2, iters=iters)
+    assert_bitexact(gpu_run(s, 2), oracle_run(orc, s, 2))
+    assert_bitexact(gpu_run(s, 2, combined=True), oracle_run(orc, s, 2))

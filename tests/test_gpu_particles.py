@@ -285,3 +285,23 @@ def test_lists_by_position_bitexact(spread):
     if spread < 3:
         per_wave = np.add.reduceat(oc.astype(np.int64), np.arange(0, n, 64))
         assert per_wave.max() > 2 * 3072  # three 3072-slot windows for some waves
+
+
+def test_density_counts_with_a_non_finite_position():
+    """A NaN position (its cell key clamps into the grid) is nobody's
+    neighbour and has none itself: the density pass's counts equal the
+    counting pass's (0 for it), and lists made from those counts equal the
+    lists the counting pass makes."""
+    x, y = _positions(3000, 100.0, 100.0, seed=77)
+    x[17], y[17] = np.nan, np.nan
+    x[1234] = np.inf
+    ng = particles.NeighborGrid(len(x), 100.0, 100.0, 1.5)
+    ng.build(torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda())
+    counts, offsets, nbr = (t.clone() for t in ng.neighbors())
+    _, dcounts = ng.density(1.0, with_counts=True)
+    dcounts = dcounts.clone()
+    np.testing.assert_array_equal(_u32(dcounts), _u32(counts))
+    assert _u32(counts)[17] == 0 and _u32(counts)[1234] == 0
+    c2, o2, n2 = ng.neighbors(counted=True)
+    np.testing.assert_array_equal(_u32(o2), _u32(offsets))
+    np.testing.assert_array_equal(_u32(n2), _u32(nbr))

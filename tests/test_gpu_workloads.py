@@ -1,0 +1,151 @@
+"""Each workload the library is measured on, run at its own size in a test:
+the precision form at 2048x2048, the 256-scene batched form, the served form
+at its job sizes and the 1M-tracer particle step.  Smaller cases of every
+form are covered by the other GPU tests; these check that nothing changes at
+the sizes the measurements quote (tile counts, chunking, table sizes, plane
+sizes), against the CPU oracle or the FP64 twin.
+
+The reference steps one scene per process (src/main.cpp:219-240) with
+simulate_fluid_step and advect_color_step (src/fluid.cpp:298-320)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from probabilistic_fluid_simulation_amd import fluid, scenes, serve  # noqa: E402
+from probabilistic_fluid_simulation_amd.batch import Batch  # noqa: E402
+from probabilistic_fluid_simulation_amd.fluid import VPField  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _residual64(q):
+    """||b + pL + pR + pT + pB - 4p|| and ||b|| in float64 over a (H, W, 4)
+    field holding p in channel 2 and b in channel 3 (periodic)."""
+    p = q[..., 2].astype(np.float64)
+    b = q[..., 3].astype(np.float64)
+    s = (np.roll(p, 1, 1) + np.roll(p, -1, 1)) + (np.roll(p, 1, 0) + np.roll(p, -1, 0))
+    r = b + s - 4 * p
+    return float(np.sqrt((r * r).sum())), float(np.sqrt((b * b).sum()))
+
+
+def test_precision_form_at_2048():
+    """The precision workload (2048x2048, 4 Taylor-Green cells, seed 0, dt =
+    1/60, nu = 0.1, at most 16 sweeps, tol 1e-4), two steps.  Each step:
+    Jacobi does not reach tol in 16 sweeps here (every residual of the
+    history stays above tol ||b||), so all 16 run; the velocity is within
+    1e-5 (max error / max magnitude) of the FP64 twin stepping the same
+    state; the reported norms equal an FP64 brute force over the output."""
+    s = scenes.precision_4m(steps=2)
+    vp, tmp = VPField.from_numpy(s.vp), VPField.from_numpy(s.tmp)
+    for step in range(2):
+        before = (vp.data.clone(), tmp.data.clone())
+        rep = fluid.simulate_fluid_step_precise(vp, tmp, s.dt, s.visc, max_iters=16, tol=1e-4)
+        assert rep["iters_used"] == 16, step
+        hist = rep["residual_history"]
+        assert all(h > 1e-4 * rep["b_norm"] for h in hist[1:16]), step
+        r, bn = _residual64(tmp.numpy())
+        assert rep["residual_norm"] == pytest.approx(r, rel=1e-6), step
+        assert rep["b_norm"] == pytest.approx(bn, rel=1e-6), step
+        v64, t64 = before[0].double(), before[1].double()
+        fluid.simulate_fluid_step_f64(v64, t64, s.dt, s.visc, 16, 16)
+        ref = v64[..., :2]
+        err = ((vp.data[..., :2].double() - ref).abs().amax(dim=(0, 1))
+               / ref.abs().amax(dim=(0, 1)))
+        assert float(err.max()) <= 1e-5, (step, err)
+
+
+def _oracle_end(orc, bufs, job, steps):
+    host = [b.data.cpu().numpy().copy() for b in bufs]
+    return orc.run(*host, job.dt, job.visc, job.iters, steps)
+
+
+def test_batched_form_256_scenes(orc):
+    """The batched workload's 256 scenes (dye 4K-256K px, field 64^2-512^2,
+    scene s from seed s, K = 8) in one Batch for 3 steps; 16 scenes, four of
+    each field size, end bit-identical to the oracle stepping them alone."""
+    specs = scenes.batched_specs(256)
+    jobs = [serve.Job(job=sp["scene"], arrival=0.0, fw=sp["fw"], fh=sp["fh"], iw=sp["iw"],
+                      ih=sp["ih"], steps=3, seed=sp["seed"], iters=8) for sp in specs]
+    sample = []
+    for side in (64, 128, 256, 512):
+        sample += [j for j in jobs if j.fw == side][:4]
+    assert len(sample) == 16
+    inputs = {j.job: None for j in sample}
+    b = Batch()
+    slots = {}
+    for j in jobs:
+        bufs = serve.device_scene(j)
+        if j.job in inputs:
+            inputs[j.job] = [t.data.cpu().numpy().copy() for t in bufs]
+        slots[j.job] = b.add(*bufs, j.dt, j.visc, j.iters)
+    assert b.count == 256
+    b.step(3)
+    b.sync()
+    for j in sample:
+        want = orc.run(*[a.copy() for a in inputs[j.job]], j.dt, j.visc, j.iters, 3)
+        got = [t.cpu().numpy() for t in b.buffers(slots[j.job])]
+        for name, g, w in zip(("vp", "tmp", "image", "itmp"), got, want):
+            assert g.tobytes() == w.tobytes(), (j.job, j.fw, j.iw, name)
+    b.close()
+
+
+def test_served_form_at_job_sizes(orc):
+    """The served workload's job stream (Poisson 64 jobs/s, seed 7, dye 8K-1M
+    px, field side matched, K = 4) at its real sizes, 16 jobs, steps cut to
+    10-20 so the oracle keeps up: every job's final dye, taken from the
+    server as it retires the job, is bit-identical to the oracle running the
+    job's inputs alone."""
+    specs = scenes.served_specs(16, rate=64.0, seed=7)
+    for sp in specs:
+        sp.update(steps=10 + sp["steps"] % 11)
+    jobs = serve.jobs_from_specs(specs, iters=4)
+    assert max(j.fw * j.fh for j in jobs) > 500_000  # the large end of the stream is in
+    srv = serve.Server(max_substeps=4, keep_results=True)
+    out = srv.run(jobs)
+    srv.close()
+    assert out["completed"] == len(jobs) and out["rejected"] == 0
+    for j in jobs:
+        bufs = serve.device_scene(j)  # the same inputs, made again
+        want = _oracle_end(orc, bufs, j, j.steps)
+        assert j.result.cpu().numpy().tobytes() == want[2].tobytes(), (j.job, j.fw, j.steps)
+
+
+def test_particle_step_at_1m_tracers():
+    """One step of the particle workload (1,048,576 tracers over the 1024^2
+    blob image, radius 1.5 cells): advect through the vortex field, rebuild
+    the neighbour grid in cell order, lists and densities, all bit-exact to
+    the particle oracle doing the same."""
+    from oracle import particles as op
+    from probabilistic_fluid_simulation_amd import particles
+
+    s, x, y, r = scenes.particles_1m()
+    H, W = s.vp.shape[:2]
+    vp = VPField.from_numpy(s.vp)
+    tr = particles.Tracers(torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda(), float(W),
+                           float(H), r)
+    n = len(x)
+    tr.advect(vp, s.dt)
+    tr.sort()
+    rho, counts_d = tr.density(1.0 / n, with_counts=True)
+    rho, counts_d = rho.clone(), counts_d.clone()  # the grid's buffers, reused below
+    counts, offsets, nbr = tr.neighbors()
+    torch.cuda.synchronize()
+    xo, yo = x.copy(), y.copy()
+    op.advect(xo, yo, s.vp, s.dt)
+    g = tr.grid.g
+    og = op.Grid(xo, yo, float(W), float(H), r, g.gx, g.gy, g.kx, g.ky)
+    u32 = lambda t: t.cpu().numpy().view(np.uint32)  # noqa: E731
+    assert tr.x.cpu().numpy().tobytes() == og.xs.tobytes()
+    assert tr.y.cpu().numpy().tobytes() == og.ys.tobytes()
+    np.testing.assert_array_equal(u32(tr.ids), og.order)
+    np.testing.assert_array_equal(u32(tr.grid.cell_start), og.start)
+    np.testing.assert_array_equal(u32(tr.grid.cell_end), og.end)
+    oc, oo, on = og.neighbors(by_position=True)
+    np.testing.assert_array_equal(u32(counts), oc)
+    np.testing.assert_array_equal(u32(counts_d), oc)
+    np.testing.assert_array_equal(u32(offsets), oo)
+    np.testing.assert_array_equal(u32(nbr), on)
+    assert rho.cpu().numpy().tobytes() == og.density(1.0 / n, by_position=True).tobytes()

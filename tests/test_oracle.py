@@ -56,6 +56,34 @@ def test_golden_advect_color(orc, ops_gold):
     assert same(out, g["advect_color_out"])
 
 
+def test_color_keys_restatement_reproduces_golden_dye(ops_gold):
+    """The numpy restatement of advect_color's keys (used to check the
+    library's key dump) gives the reference's own advect_color output when
+    the blend is taken at those keys: golden vectors from src/fluid.cpp at a
+    17x13 field and a 23x19 dye."""
+    from keys_util import bilerp_from_keys, color_keys_numpy
+
+    g = ops_gold
+    shape = g["image"].shape[:2]
+    keys = color_keys_numpy(shape, g["vp"], float(g["dt"]))
+    assert same(bilerp_from_keys(g["image"], keys, shape, g["vp"], float(g["dt"])),
+                g["advect_color_out"])
+
+
+@pytest.mark.parametrize("shape", [(17, 13, 23, 19), (40, 30, 90, 45), (64, 64, 32, 32)])
+def test_color_keys_restatement_reproduces_oracle_dye(orc, shape):
+    """Same pin on random states with multi-pixel back-traces, against the
+    oracle (itself bit-exact to the reference on the golden vectors)."""
+    from keys_util import bilerp_from_keys, color_keys_numpy
+
+    fw, fh, iw, ih = shape
+    s = scenes.random_scene(fw, fh, iw, ih, seed=31, dt=0.8, vel_scale=9.0 * iw)
+    keys = color_keys_numpy((ih, iw), s.vp, s.dt)
+    out = np.zeros_like(s.image)
+    orc.advect_color(s.image.copy(), out, s.vp.copy(), s.dt)
+    assert same(bilerp_from_keys(s.image, keys, (ih, iw), s.vp, s.dt), out)
+
+
 def test_golden_gradient(orc, ops_gold):
     g = ops_gold
     out = g["tmp"].copy()
@@ -318,3 +346,31 @@ def test_double_twin_tracks_float(orc, orc64):
     # (src/fluid.cpp:48), which the double twin does not: the dye differs
     # most at the disk's sharp edge.
     np.testing.assert_allclose(out32[2], out64[2], atol=1e-3)
+
+
+# ------------------------------------------------------------- force injection
+def test_add_forces_apply_constant_force_is_uniform_shift(anyorc):
+    """A constant force on a constant velocity field shifts every cell by the
+    same dt*f (closed form; channels past fz untouched)."""
+    dt = 0.0625
+    vp = np.empty((13, 17, 4), dtype=anyorc.b.dtype)
+    vp[..., 0], vp[..., 1], vp[..., 2], vp[..., 3] = 0.25, -0.5, 3.0, 7.0
+    f = np.empty((13, 17, 2), dtype=anyorc.b.dtype)
+    f[..., 0], f[..., 1] = 2.0, -4.0
+    anyorc.add_forces_apply(vp, f, dt)
+    assert np.all(vp[..., 0] == 0.25 + 0.0625 * 2.0) and np.all(vp[..., 1] == -0.5 - 0.0625 * 4.0)
+    assert np.all(vp[..., 2] == 3.0) and np.all(vp[..., 3] == 7.0)
+
+
+def test_add_forces_apply_elementwise(orc):
+    """Random fields: each element is x + RN(dt*f) in float32, the first fz
+    channels only, with the forces' own [y][x][fz] layout."""
+    rng = np.random.default_rng(5)
+    for fz in (1, 2, 3):
+        vp = rng.standard_normal((9, 11, 4)).astype(np.float32)
+        f = rng.standard_normal((9, 11, fz)).astype(np.float32)
+        want = vp.copy()
+        dt = np.float32(0.37)
+        want[..., :fz] = vp[..., :fz] + dt * f
+        orc.add_forces_apply(vp, f, float(dt))
+        assert same(vp, want)
