@@ -53,6 +53,23 @@ def pmc_figure(name: str, kernel: str):
     return None
 
 
+def trace_us(phase: str, kernel: str):
+    """Average duration in µs of `kernel` in `phase` from the committed
+    rocprofv3 kernel-trace summary of this workload (its "By phase" table),
+    or None."""
+    path = os.path.join(ROOT, "profiles", "plain-1024x1024-K8_kernel_trace.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 4 and cells[0] == phase and cells[1] == kernel:
+            try:
+                return float(cells[3])
+            except ValueError:
+                return None
+    return None
+
+
 # VALU issue rate: a wave64 FP32 instruction holds a 16-lane SIMD for 4 cycles;
 # 256 CUs x 4 SIMDs at 2.4 GHz.
 VALU_WAVE_INSTS_PER_S = 256 * 4 * 2.4e9 / 4
@@ -92,8 +109,8 @@ def cpu_baseline(scene, budget_steps: int) -> dict:
     dt = timed(run, budget_steps)
     out = {"value": scene.cells * budget_steps / dt, "unit": "particle-steps/s", "cores": 1,
            "kind": kind,
-           "sample": f"{scene.name}: first {budget_steps} of {scene.steps} steps, {what}, "
-                     f"{dt:.1f} s",
+           "sample": f"{scene.name}: the first {budget_steps} of the workload's {scene.steps} "
+                     f"steps, {what}, {dt:.1f} s",
            "nproc": os.cpu_count()}
 
     threads = orc.max_threads()
@@ -354,6 +371,21 @@ def precision_leg(steps: int, sample_every: int, side: int = 2048) -> dict:
             scale = ref.abs().amax(dim=(0, 1))
             err_c.append(float(((vp.data[..., :2].double() - ref).abs().amax(dim=(0, 1)) / scale).max()))
             err_p.append(float(((pv.data[..., :2].double() - ref).abs().amax(dim=(0, 1)) / scale).max()))
+    # The launches' schedule (sweeps per launch, tile side) on copies of the
+    # current state, 10 steps each; results do not depend on it.
+    sched_ms = {}
+    for sched in ((4, 64), (8, 64), (4, 32), (8, 32)):
+        fluid.set_precise_schedule(*sched)
+        sv, st = fluid.VPField(vp.data.clone()), fluid.VPField(tmp.data.clone())
+        fluid.simulate_fluid_step_precise(sv, st, s.dt, s.visc, 16, 1e-4, report=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(10):
+            fluid.simulate_fluid_step_precise(sv, st, s.dt, s.visc, 16, 1e-4, report=False)
+        torch.cuda.synchronize()
+        sched_ms[f"{sched[0]}x{sched[1]}"] = (time.perf_counter() - t0) / 10 * 1e3
+        del sv, st
+    fluid.set_precise_schedule(0, 0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -369,7 +401,8 @@ def precision_leg(steps: int, sample_every: int, side: int = 2048) -> dict:
             "final_rel_residual_max": max(rel_res),
             "velocity_rel_err_vs_fp64_max": max(err_c),
             "plain_fp32_velocity_rel_err_vs_fp64_max": max(err_p),
-            "error_samples": len(err_c)}
+            "error_samples": len(err_c),
+            "schedule_ms_per_step": sched_ms}
 
 
 def density_fp64_error(xs, ys, counts, nbr, rho, mass, r, W, H) -> float:
@@ -579,6 +612,115 @@ def served_leg(rate: float, duration: float, seed: int = 7) -> dict:
     return out
 
 
+def host_flavour_leg(steps: int = 5) -> dict:
+    """The 1M workload through the fluid.hpp host flavour, called by its C++
+    symbols as a recompiled src/main.cpp CPU branch would (simulate_fluid_step
+    then advect_color_step on vp_fields over host memory, :236-239): each
+    call copies what it reads to the GPU's mirrors of the buffers and what it
+    writes back.  One untimed timestep (the mirrors are made), then `steps`
+    timed; the end state must equal the device path's."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    from probabilistic_fluid_simulation_amd import _lib, fluid, scenes
+
+    class VpField(ctypes.Structure):  # includes/fluid.hpp:17-22
+        _fields_ = [("x", ctypes.c_int), ("y", ctypes.c_int), ("z", ctypes.c_int),
+                    ("data", ctypes.c_void_p)]
+
+    lib = _lib.load()
+    step = lib._Z19simulate_fluid_stepP8vp_fieldS0_ff
+    step.argtypes, step.restype = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float,
+                                   ctypes.c_float], None
+    dye = lib._Z17advect_color_stepP8vp_fieldS0_S0_f
+    dye.argtypes, dye.restype = [ctypes.c_void_p] * 3 + [ctypes.c_float], None
+    setk = lib._Z20set_num_jacobi_itersi
+    setk.argtypes, setk.restype = [ctypes.c_int], None
+    s = scenes.plain_1m()
+    arrays = [a.copy() for a in (s.vp, s.tmp, s.image, s.itmp)]
+    f = [VpField(a.shape[1], a.shape[0], a.shape[2], a.ctypes.data) for a in arrays]
+    by_addr = {a.ctypes.data: a for a in arrays}
+    setk(s.iters)
+
+    def one():
+        step(ctypes.byref(f[0]), ctypes.byref(f[1]), s.dt, s.visc)
+        dye(ctypes.byref(f[2]), ctypes.byref(f[3]), ctypes.byref(f[0]), s.dt)
+
+    one()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        one()
+    t = (time.perf_counter() - t0) / steps
+    setk(30)
+    got = [by_addr[x.data] for x in f]
+    dev = [fluid.VPField.from_numpy(a) for a in (s.vp, s.tmp, s.image, s.itmp)]
+    for _ in range(steps + 1):
+        fluid.fluid_timestep(*dev, s.dt, s.visc, s.iters)
+    torch.cuda.synchronize()
+    same = all(np.asarray(g).tobytes() == d.numpy().tobytes() for g, d in zip(got, dev))
+    return {"workload": s.name, "steps": steps, "host_flavour_ms_per_step": t * 1e3,
+            "bytes_copied_per_step": 2 * 2 * s.vp.nbytes + 2 * s.image.nbytes + s.vp.nbytes,
+            "end_state_equals_device_path": bool(same),
+            "path": "fluid.hpp vp_field flavour on host buffers (device mirrors kept "
+                    "between calls; PCIe copies of what each call reads and writes)"}
+
+
+def served_capacity_leg(n_jobs: int, seed: int = 7) -> dict:
+    """The served form saturated: the served workload's first n_jobs jobs
+    (dye 8K-1M px, field side matched, 50-500 steps, K = 4, dt = 1/60, nu =
+    0.1, seed 7) all arriving at once, through the server (admission,
+    continuous batching, inputs made on the device at admission, results
+    copied out at retirement) -> the most jobs/s it completes.  Beside it
+    the same jobs one after another on one stream, inputs made beforehand
+    and not timed: with a launch per kernel per step (pfs_fluid_timestep),
+    and with each job's steps replayed from hipGraphs (pfs_fluid_run).  The
+    server's dye results must equal the per-step loop's."""
+    import torch
+
+    from probabilistic_fluid_simulation_amd import fluid, scenes, serve
+
+    specs = scenes.served_specs(n_jobs, rate=1.0, seed=seed)
+    for sp in specs:
+        sp["arrival"] = 0.0
+    warm = serve.Server()
+    warm.run(serve.jobs_from_specs(scenes.served_specs(4, 1000.0, seed=seed + 1), iters=4))
+    warm.close()
+    jobs = serve.jobs_from_specs(specs, iters=4)
+    srv = serve.Server(keep_results=True)
+    out = srv.run(jobs)
+    out["peak_active"] = srv.peak_active
+    srv.close()
+
+    def loop(run):
+        bufs = [serve.device_scene(j) for j in jobs]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for j, b in zip(jobs, bufs):
+            if run:
+                fluid.fluid_run(*b, j.dt, j.visc, j.iters, j.steps)
+            else:
+                for _ in range(j.steps):
+                    fluid.fluid_timestep(*b, j.dt, j.visc, j.iters)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, bufs
+
+    t_plain, bufs = loop(False)
+    same = all(torch.equal(j.result, b[2].data) for j, b in zip(jobs, bufs))
+    del bufs
+    t_run, _ = loop(True)
+    cells = sum(j.fw * j.fh * j.steps for j in jobs)
+    return {"jobs": n_jobs, "arrivals": "all at t = 0", "server_jobs_per_s": out["jobs_per_s"],
+            "server_makespan_s": out["makespan_s"], "peak_active": out["peak_active"],
+            "p50_latency_s": out["p50_latency_s"], "p99_latency_s": out["p99_latency_s"],
+            "plain_loop_jobs_per_s": n_jobs / t_plain, "run_loop_jobs_per_s": n_jobs / t_run,
+            "speedup_vs_plain_loop": t_plain / out["makespan_s"],
+            "speedup_vs_run_loop": t_run / out["makespan_s"],
+            "cell_steps": cells, "server_cell_steps_per_s": cells / out["makespan_s"],
+            "results_identical_to_plain_loop": bool(same)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -593,6 +735,7 @@ def main() -> int:
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-tiles", action="store_true")
     ap.add_argument("--no-small-scene", action="store_true")
+    ap.add_argument("--no-host-flavour", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-particles", action="store_true")
     ap.add_argument("--particle-steps", type=int, default=50)
@@ -600,6 +743,8 @@ def main() -> int:
     ap.add_argument("--served-rates", default="64",
                     help="comma-separated arrival rates (jobs/s) of the served leg")
     ap.add_argument("--served-seconds", type=float, default=6.0)
+    ap.add_argument("--served-capacity-jobs", type=int, default=96,
+                    help="jobs of the saturated served run (0: skip it)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, one device-to-device copy of this many MiB (a "
                          "kernel of known traffic for calibrating HBM byte counters)")
@@ -700,6 +845,13 @@ def main() -> int:
                                        "WRITE_SIZE x1, calibrated on a 256 MiB copy)",
                      "algorithmic_bytes_per_launch": alg[dom]},
     }
+    rp_us = trace_us("timed loop", "k_diffuse" if dom == "diffuse" else "k_pressure + dye")
+    if rp_us is not None:
+        # The same kernel's average duration in the timed loop under rocprofv3
+        # (committed profile), beside the event-derived one used above.
+        out["roofline"]["rocprof_kernel_us"] = rp_us
+        out["roofline"]["rocprof_frac"] = alg[dom] / (rp_us * 1e-6) / 1e9 / HBM_PEAK_GBS
+        out["roofline"]["kernel_us_events"] = avg_ms[dom] * 1e3
     valu = pmc_figure("valu_insts_per_launch", dom)
     if valu is not None:
         # The kernel's other bound: the time its VALU instructions take to
@@ -707,7 +859,7 @@ def main() -> int:
         out["roofline"]["valu_issue_frac"] = valu / VALU_WAVE_INSTS_PER_S / (avg_ms[dom] * 1e-3)
         out["roofline"]["valu_insts_per_launch"] = valu
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(scenes.plain_1m(steps=args.steps), args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(scenes.plain_1m(), args.cpu_steps)
         out["cpu_baseline"]["whole_run_check"] = whole_run_check(s, end_state,
                                                                  args.warmup + args.steps)
     del vp, tmp, im, itmp
@@ -717,9 +869,13 @@ def main() -> int:
         out["tiles"] = tiles_leg()
     if not args.no_small_scene:
         out["small_scene"] = small_scene_leg()
+    if not args.no_host_flavour:
+        out["host_flavour"] = host_flavour_leg()
     if not args.no_served:
         out["served"] = [served_leg(float(r), args.served_seconds)
                          for r in args.served_rates.split(",")]
+        if args.served_capacity_jobs > 0:
+            out["served_capacity"] = served_capacity_leg(args.served_capacity_jobs)
     if not args.no_particles:
         out["particles"] = particles_leg(args.particle_steps)
     if not args.no_precision:

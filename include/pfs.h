@@ -245,9 +245,12 @@ int pfs_batch_autotune(pfs_batch *b, int reps, int *side, float *ms, int n_ms);
  * pfs_simulate_fluid_step with K = max_iters diffusion sweeps, except:
  * compensated (twice-FP32) stencil sums in both sweeps; an FP64 residual
  * ||b + pL + pR + pT + pB - 4p|| of every pressure iterate; and the pressure
- * iteration stops at the first iterate with residual <= tol * ||b|| (Kp <=
- * max_iters sweeps), decided on the device.  With Kp = K the result has the
- * reference's layout: tmp = (E, p^K, b), vp = (E - grad p^K, p^(K-1), b). */
+ * iteration stops at the first iterate i >= 1 with residual <= tol * ||b||
+ * (Kp <= max_iters sweeps), decided on the device.  With Kp = K the result
+ * has the reference's layout: tmp = (E, p^K, b), vp = (E - grad p^K,
+ * p^(K-1), b).  The FP64 sums are folded in a fixed order: the report, the
+ * exit and the outputs are the same on every run.  vp and tmp must be
+ * 16-byte aligned. */
 #define PFS_PRECISE_MAX_ITERS 64
 typedef struct pfs_precise_report {
     int iters_used;                                    /* Kp */
@@ -263,6 +266,10 @@ int pfs_simulate_fluid_step_precise(pfs_field *vp, pfs_field *tmp, float dt, flo
                                     int max_iters, double tol, void *workspace,
                                     size_t workspace_bytes, void *stream,
                                     pfs_precise_report *report);
+/* Sweeps per launch of the precision form's temporally blocked kernels (4
+ * or 8; 0: the default, 8) and their tile side (16, 32, 64; 0: the plain
+ * step's rule), process-wide.  Results do not depend on either. */
+int pfs_set_precise_schedule(int sweeps_per_launch, int tile);
 /* FP64 twin: the same step and contract on double [vy][vx][4] device buffers
  * with plain arithmetic and given sweep counts (1 <= iters_pressure <=
  * iters_diffuse); with equal counts it is the reference's algorithm in

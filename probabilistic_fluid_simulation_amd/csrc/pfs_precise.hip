@@ -5,143 +5,75 @@
 // step structure (src/fluid.cpp:298-310) and changes three things:
 //
 //   - the 5-term stencil sums of both Jacobi sweeps (src/fluid.cpp:19-21)
-//     are compensated: products by TwoProduct (FMA), sums by TwoSum, so the
-//     sum is the float nearest its value in about twice FP32 precision
-//     (Ogita, Rump & Oishi 2005, Dot2/Sum2) before the single rounding and
-//     the division;
+//     are compensated: sums by TwoSum with their errors carried, the product
+//     by alpha with its exact FMA error (Ogita, Rump & Oishi 2005,
+//     Sum2/Dot2), so each numerator is rounded once from about twice FP32
+//     precision before the division;
 //   - every pressure sweep also accumulates, in FP64, the residual of its
 //     input iterate, r = b + pL + pR + pT + pB - 4p (zero at the fixed point
-//     of the sweep p' = (pL+pR+pT+pB+b)/4), and sum b^2 once;
-//   - the pressure iteration stops after the first iterate whose residual
-//     satisfies ||r|| <= tol ||b||, at most K sweeps.  The test runs on the
-//     device: sweep s reads the completed residual of iterate s-2 and skips
-//     itself once converged, so no sweep waits on the host.
+//     of the sweep p' = (pL+pR+pT+pB+b)/4), from the compensated sum, and
+//     sum b^2 once;
+//   - the pressure iteration stops at the first iterate i >= 1 with
+//     ||r(p^i)|| <= tol ||b||, at most K sweeps, decided on the device.
+//
+// Kernels (temporally blocked like the plain step, pfs_step.hip, on the
+// same register-strip tiles, pfs_step_kernels.h):
+//
+//   k_pctl_clear          control block of the step
+//   k_diffuse_precise     advect + C diffusion sweeps per launch (compensated),
+//                         the last launch also the divergence and the
+//                         pressure start (diffuse_tile<..., COMP = true>)
+//   k_pressure_precise    C pressure sweeps per launch from an iterate plane,
+//                         with the FP64 residual of every iterate the launch
+//                         reads on the tile's own cells; launch `launch`
+//                         covers iterates launch*C .. launch*C + C.
+//
+// Every workgroup writes its per-row FP64 sums to its own slot (one per
+// tile); the last workgroup of a launch to finish folds each row over the
+// tiles in tile order (a fixed order: the totals, and so the early exit, do
+// not depend on scheduling) and decides.  The next launch reads that
+// decision at entry:
+//   - no iterate converged, more sweeps to go: continue from the plane the
+//     last launch wrote (planes rotate by launch: launch k reads k % 3 and
+//     writes (k+1) % 3);
+//   - iterate i converged: re-run the sweeps that end at i from the plane
+//     that precedes them, still intact, then the gradient subtraction and the
+//     outputs; later launches return at entry.
+// The last chunk writes the outputs for Kp = K as it goes; one more launch
+// fixes them up when the fold of that chunk finds an earlier converged
+// iterate.  So the early exit costs no host round trip.
 //
 // Buffer contract, with Kd = K the diffusion sweeps and Kp <= K the pressure
-// sweeps actually run: E = D^Kd if Kd is even else D^(Kd-1); the pressure
+// sweeps of the result: E = D^Kd if Kd is even else D^(Kd-1); the pressure
 // starts from tmp[2] if Kd is even else vp[2];
 //   tmp = (E.u, E.v, p^Kp, b)     vp = (E - grad p^Kp, p^(Kp-1), b)
 // which is exactly the reference's step when Kp = Kd (the plain step's
 // contract, see pfs_step.hip).
 //
-// The FP64 twin runs the same contract in double with plain arithmetic and
-// given (Kd, Kp): the yardstick the precision form's error is reported
-// against.  With Kd = Kp = K it reproduces the reference's algorithm in
-// double exactly.
+// The FP64 twin runs the same contract in double with plain arithmetic, one
+// launch per sweep and given (Kd, Kp): the yardstick the precision form's
+// error is reported against.  With Kd = Kp = K it reproduces the
+// reference's algorithm in double exactly.
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <string>
 
 #include "pfs.h"
 #include "pfs_internal.h"
+#include "pfs_step_kernels.h"
 
 namespace pfs {
 namespace {
 
-constexpr int kBX = 64, kBY = 4;
-constexpr int kRows = 8;  // rows per thread: a block covers kBX x (kBY * kRows) cells
 constexpr int kMaxK = PFS_PRECISE_MAX_ITERS;
 
-template <typename T>
-struct V2;
-template <>
-struct V2<float> {
-    using type = float2;
-};
-template <>
-struct V2<double> {
-    using type = double2;
-};
-
-// FP64 sums are accumulated with one atomic per block, spread over kSlots
-// addresses a cache line apart (slot = block index mod kSlots) so that
-// blocks do not all queue on one address; a reader adds the slots in
-// slot order.
-constexpr int kSlots = 32;
-constexpr int kSlotStride = 16;     // doubles: 128 bytes
-constexpr int kSumB2 = kMaxK + 1;   // row of sum b^2
-constexpr int kSumFinal = kMaxK + 2;  // row of sum r^2 of the kept iterate
-constexpr int kSums = kMaxK + 3;    // rows 0..kMaxK: sum r^2 of pressure iterate s
-
-// Device-side state of one precision step (cleared by k_ctl_clear).
-struct Ctl {
-    double sum[kSums][kSlots * kSlotStride];
-    int done, kp;
-};
-
-// The folded totals the host reads back (written by k_ctl_fold).
-struct CtlTotals {
-    double sum[kSums];
-    int kp, pad;
-};
-
-__device__ __forceinline__ double ctl_total(const Ctl *c, int row) {
-    double t = 0.0;
-#pragma unroll 8
-    for (int k = 0; k < kSlots; ++k) t += c->sum[row][k * kSlotStride];
-    return t;
-}
-
-// ---- compensated arithmetic (error-free transformations) -------------------
-__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
-    s = a + b;
-    const float bb = s - a;
-    e = (a - (s - bb)) + (b - bb);
-}
-
-// alpha*(l + r + t + b) + c summed as written in src/fluid.cpp:175-182 but
-// compensated: the float nearest the exactly rounded-once value.
-__device__ __forceinline__ float diffuse_sum_comp(float l, float r, float t, float b, float c,
-                                                  float alpha) {
-    float s = alpha * l;
-    float err = fmaf(alpha, l, -s);
-    const float x[3] = {r, t, b};
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const float p = alpha * x[k];
-        const float ep = fmaf(alpha, x[k], -p);
-        float e;
-        two_sum(s, p, s, e);
-        err += e + ep;
-    }
-    float e;
-    two_sum(s, c, s, e);
-    err += e;
-    return s + err;
-}
-
-__device__ __forceinline__ float pressure_sum_comp(float l, float r, float t, float b, float rhs) {
-    float s = l, err = 0.0f, e;
-    two_sum(s, r, s, e);
-    err += e;
-    two_sum(s, t, s, e);
-    err += e;
-    two_sum(s, b, s, e);
-    err += e;
-    two_sum(s, rhs, s, e);
-    err += e;
-    return s + err;
-}
-
-// Block-wide FP64 sum into row `row` of the control sums (one atomic per
-// block).
-__device__ __forceinline__ void block_add(double v, Ctl *ctl, int row) {
-    __shared__ double part[kBX * kBY / 64];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    const int tid = threadIdx.y * kBX + threadIdx.x;
-    if ((tid & 63) == 0) part[tid >> 6] = v;
-    __syncthreads();
-    if (tid == 0) {
-        double s = 0.0;
-        for (int k = 0; k < kBX * kBY / 64; ++k) s += part[k];
-        const int slot = (int)((blockIdx.y * gridDim.x + blockIdx.x) % kSlots);
-        atomicAdd(&ctl->sum[row][slot * kSlotStride], s);
-    }
-}
+// ============================ FP64 twin ======================================
+constexpr int kBX = 64, kBY = 4;
+constexpr int kRows = 8;  // rows per thread: a block covers kBX x (kBY * kRows) cells
 
 struct Cell {
     int i, j, im, ip, jm, jp;
@@ -167,33 +99,17 @@ __device__ __forceinline__ void for_cells(int w, int h, F &&f) {
     }
 }
 
-// Grid (kSums): block r clears row r; block 0 also the decision.
-__global__ void k_ctl_clear(Ctl *c) {
-    for (int k = threadIdx.x; k < kSlots * kSlotStride; k += blockDim.x) c->sum[blockIdx.x][k] = 0.0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        c->done = 0;
-        c->kp = 0;
-    }
-}
-
-// One block of 128 threads: thread r < kSums folds row r.
-static_assert(kSums <= 128, "k_ctl_fold folds one row per thread");
-__global__ void k_ctl_fold(const Ctl *c, CtlTotals *t) {
-    if ((int)threadIdx.x < kSums) t->sum[threadIdx.x] = ctl_total(c, threadIdx.x);
-    if (threadIdx.x == 0) t->kp = c->kp;
-}
-
 // Advection of vp[0,1] (src/fluid.cpp:24-70) into the iterate plane d.
-template <typename T>
-__global__ __launch_bounds__(kBX *kBY) void k_p_advect(const T *__restrict__ P,
-                                                      typename V2<T>::type *__restrict__ d, int w,
-                                                      int h, T dt, T fw, T fh, T inv_w, T inv_h,
-                                                      int pow2_w, int pow2_h) {
+__global__ __launch_bounds__(kBX *kBY) void k_t_advect(const double *__restrict__ P,
+                                                      double2 *__restrict__ d, int w, int h,
+                                                      double dt, double fw, double fh,
+                                                      double inv_w, double inv_h, int pow2_w,
+                                                      int pow2_h) {
     for_cells(w, h, [&](const Cell &c) {
         const int cell = c.j * w + c.i;
-        const T xp = (T)c.i - div_n(dt * P[4 * cell], fw, inv_w, pow2_w);
-        const T yp = (T)c.j - div_n(dt * P[4 * cell + 1], fh, inv_h, pow2_h);
-        const TraceT<T> t = trace(xp, yp, w, h, fw, fh);
+        const double xp = (double)c.i - div_n(dt * P[4 * cell], fw, inv_w, pow2_w);
+        const double yp = (double)c.j - div_n(dt * P[4 * cell + 1], fh, inv_h, pow2_h);
+        const TraceT<double> t = trace(xp, yp, w, h, fw, fh);
         const int a00 = 4 * (t.j0 * w + t.i0), a10 = 4 * (t.j0 * w + t.i1);
         const int a01 = 4 * (t.j1 * w + t.i0), a11 = 4 * (t.j1 * w + t.i1);
         d[cell] = {bilerp(t.sx, t.sy, P[a00], P[a10], P[a01], P[a11]),
@@ -202,117 +118,62 @@ __global__ __launch_bounds__(kBX *kBY) void k_p_advect(const T *__restrict__ P,
 }
 
 // One diffusion sweep (src/fluid.cpp:154-185) on the iterate planes.
-template <typename T, bool COMP>
-__global__ __launch_bounds__(kBX *kBY) void k_p_diffuse(const typename V2<T>::type *__restrict__ in,
-                                                       typename V2<T>::type *__restrict__ out,
-                                                       int w, int h, T alpha, T beta) {
+__global__ __launch_bounds__(kBX *kBY) void k_t_diffuse(const double2 *__restrict__ in,
+                                                       double2 *__restrict__ out, int w, int h,
+                                                       double alpha, double beta) {
     for_cells(w, h, [&](const Cell &c) {
-        const auto L = in[c.j * w + c.im], R = in[c.j * w + c.ip], Tp = in[c.jm * w + c.i],
-                   B = in[c.jp * w + c.i], C = in[c.j * w + c.i];
-        if constexpr (COMP) {
-            out[c.j * w + c.i] = {diffuse_sum_comp(L.x, R.x, Tp.x, B.x, C.x, alpha) / beta,
-                                  diffuse_sum_comp(L.y, R.y, Tp.y, B.y, C.y, alpha) / beta};
-        } else {
-            out[c.j * w + c.i] = {diffuse_cell(L.x, R.x, Tp.x, B.x, C.x, alpha, beta),
-                                  diffuse_cell(L.y, R.y, Tp.y, B.y, C.y, alpha, beta)};
-        }
+        const double2 L = in[c.j * w + c.im], R = in[c.j * w + c.ip], T = in[c.jm * w + c.i],
+                      B = in[c.jp * w + c.i], C = in[c.j * w + c.i];
+        out[c.j * w + c.i] = {diffuse_cell(L.x, R.x, T.x, B.x, C.x, alpha, beta),
+                              diffuse_cell(L.y, R.y, T.y, B.y, C.y, alpha, beta)};
     });
 }
 
 // Divergence of D^Kd (src/fluid.cpp:221-237) into b, and the pressure start
 // (tmp[2] for even Kd, vp[2] for odd) into iterate plane 0.
-template <typename T, bool COMP>
-__global__ __launch_bounds__(kBX *kBY) void k_p_divergence(const typename V2<T>::type *__restrict__ dk,
-                                                          const T *__restrict__ P,
-                                                          const T *__restrict__ Q, T *__restrict__ b,
-                                                          T *__restrict__ p0, int w, int h,
-                                                          T gamma, int kd_even, Ctl *ctl) {
-    double b2 = 0.0;
+__global__ __launch_bounds__(kBX *kBY) void k_t_divergence(const double2 *__restrict__ dk,
+                                                          const double *__restrict__ P,
+                                                          const double *__restrict__ Q,
+                                                          double *__restrict__ b,
+                                                          double *__restrict__ p0, int w, int h,
+                                                          double gamma, int kd_even) {
     for_cells(w, h, [&](const Cell &c) {
         const int cell = c.j * w + c.i;
-        const T uR = dk[c.j * w + c.ip].x - dk[c.j * w + c.im].x;
-        const T vT = dk[c.jp * w + c.i].y - dk[c.jm * w + c.i].y;
-        const T bb = gamma * (uR + vT);
-        b[cell] = bb;
+        const double uR = dk[c.j * w + c.ip].x - dk[c.j * w + c.im].x;
+        const double vT = dk[c.jp * w + c.i].y - dk[c.jm * w + c.i].y;
+        b[cell] = gamma * (uR + vT);
         p0[cell] = kd_even ? Q[4 * cell + 2] : P[4 * cell + 2];
-        b2 += (double)bb * (double)bb;
     });
-    if constexpr (COMP) block_add(b2, ctl, kSumB2);
 }
 
-// Pressure sweep s (src/fluid.cpp:240-257): p^s from p^(s-1).  With COMP,
-// also the FP64 residual of p^(s-1) into row s-1, and the convergence test
-// of iterate s-2, whose residual the previous sweep completed.
-template <typename T, bool COMP>
-__global__ __launch_bounds__(kBX *kBY) void k_p_pressure(const T *__restrict__ in,
-                                                        T *__restrict__ out,
-                                                        const T *__restrict__ b, int w, int h, int s,
-                                                        double tol2, Ctl *ctl) {
-    if constexpr (COMP) {
-        // One thread decides for the block; every block reaches the same
-        // verdict from the same completed sums, added in the same order.
-        __shared__ int skip;
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            const bool converged =
-                s >= 3 && ctl_total(ctl, s - 2) <= tol2 * ctl_total(ctl, kSumB2);
-            skip = ctl->done || converged;
-            if (converged && !ctl->done) {
-                ctl->kp = s - 2;
-                ctl->done = 1;
-            }
-        }
-        __syncthreads();
-        if (skip) return;
-    }
-    double r2 = 0.0;
+// Pressure sweep (src/fluid.cpp:240-257): p^s from p^(s-1).
+__global__ __launch_bounds__(kBX *kBY) void k_t_pressure(const double *__restrict__ in,
+                                                        double *__restrict__ out,
+                                                        const double *__restrict__ b, int w,
+                                                        int h) {
     for_cells(w, h, [&](const Cell &c) {
-        const int cell = c.j * w + c.i;
-        const T pL = in[c.j * w + c.im], pR = in[c.j * w + c.ip];
-        const T pT = in[c.jm * w + c.i], pB = in[c.jp * w + c.i];
-        const T bb = b[cell];
-        if constexpr (COMP) {
-            out[cell] = pressure_sum_comp(pL, pR, pT, pB, bb) * 0.25f;
-            const double r = (double)bb + (double)pL + (double)pR + (double)pT + (double)pB -
-                             4.0 * (double)in[cell];
-            r2 += r * r;
-        } else {
-            out[cell] = pressure_cell(pL, pR, pT, pB, bb);
-        }
+        out[c.j * w + c.i] = pressure_cell(in[c.j * w + c.im], in[c.j * w + c.ip],
+                                           in[c.jm * w + c.i], in[c.jp * w + c.i],
+                                           b[c.j * w + c.i]);
     });
-    if constexpr (COMP) block_add(r2, ctl, s - 1);
 }
 
 // Gradient subtraction with p^Kp (src/fluid.cpp:269-296) and the output
-// contract.  COMP: Kp comes from the device-side test (with the last
-// iterate's residual checked here); otherwise kp_given.
-template <typename T, bool COMP>
-__global__ __launch_bounds__(kBX *kBY) void k_p_finalize(
-    T *__restrict__ P, T *__restrict__ Q, const typename V2<T>::type *__restrict__ dk,
-    const typename V2<T>::type *__restrict__ dkm1, const T *p0, const T *p1, const T *p2,
-    const T *__restrict__ b, int w, int h, T dt, int kd, int kp_given, double tol2, Ctl *ctl) {
-    int kp = kp_given;
-    if constexpr (COMP) {
-        __shared__ int kp_s;
-        if (threadIdx.x == 0 && threadIdx.y == 0)
-            kp_s = ctl->done ? ctl->kp
-                             : ((kd >= 2 && ctl_total(ctl, kd - 1) <= tol2 * ctl_total(ctl, kSumB2))
-                                    ? kd - 1
-                                    : kd);
-        __syncthreads();
-        kp = kp_s;
-    }
-    // Iterate k lives in plane k % 3.
-    const T *pk = kp % 3 == 0 ? p0 : kp % 3 == 1 ? p1 : p2;
-    const T *pkm1 = kp % 3 == 1 ? p0 : kp % 3 == 2 ? p1 : p2;
-    double r2 = 0.0;
+// contract; iterate k lives in plane k % 3.
+__global__ __launch_bounds__(kBX *kBY) void k_t_finalize(
+    double *__restrict__ P, double *__restrict__ Q, const double2 *__restrict__ dk,
+    const double2 *__restrict__ dkm1, const double *p0, const double *p1, const double *p2,
+    const double *__restrict__ b, int w, int h, double dt, int kd, int kp) {
+    const double *pk = kp % 3 == 0 ? p0 : kp % 3 == 1 ? p1 : p2;
+    const double *pkm1 = kp % 3 == 1 ? p0 : kp % 3 == 2 ? p1 : p2;
     for_cells(w, h, [&](const Cell &c) {
         const int cell = c.j * w + c.i;
-        const T pL = pk[c.j * w + c.im], pR = pk[c.j * w + c.ip];
-        const T pT = pk[c.jm * w + c.i], pB = pk[c.jp * w + c.i];
-        const T gradX = (pR - pL) * dt / (T)2.0f;
-        const T gradY = (pB - pT) * dt / (T)2.0f;
-        const auto e = (kd % 2 == 0) ? dk[cell] : dkm1[cell];
-        const T bb = b[cell];
+        const double pL = pk[c.j * w + c.im], pR = pk[c.j * w + c.ip];
+        const double pT = pk[c.jm * w + c.i], pB = pk[c.jp * w + c.i];
+        const double gradX = (pR - pL) * dt / 2.0;
+        const double gradY = (pB - pT) * dt / 2.0;
+        const double2 e = (kd % 2 == 0) ? dk[cell] : dkm1[cell];
+        const double bb = b[cell];
         P[4 * cell] = e.x - gradX;
         P[4 * cell + 1] = e.y - gradY;
         P[4 * cell + 2] = pkm1[cell];
@@ -321,57 +182,418 @@ __global__ __launch_bounds__(kBX *kBY) void k_p_finalize(
         Q[4 * cell + 1] = e.y;
         Q[4 * cell + 2] = pk[cell];
         Q[4 * cell + 3] = bb;
-        const double r = (double)bb + (double)pL + (double)pR + (double)pT + (double)pB -
-                         4.0 * (double)pk[cell];
-        r2 += r * r;
     });
-    if constexpr (COMP) {
-        block_add(r2, ctl, kSumFinal);
-        if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0)
-            ctl->kp = kp;
+}
+
+// Workspace of the twin: two iterate planes of double2, b and three pressure
+// planes of double.
+size_t twin_ws_bytes(int w, int h) { return (size_t)w * h * (2 * sizeof(double2) + 4 * sizeof(double)); }
+
+int twin_step(double *P, double *Q, int w, int h, double dt, double visc, int kd, int kp, void *ws,
+              hipStream_t s) {
+    const size_t N = (size_t)w * h;
+    double2 *d[2] = {static_cast<double2 *>(ws), static_cast<double2 *>(ws) + N};
+    double *b = reinterpret_cast<double *>(d[1] + N);
+    double *p[3] = {b + N, b + 2 * N, b + 3 * N};
+    const dim3 grid((w + kBX - 1) / kBX, (h + kBY * kRows - 1) / (kBY * kRows)), block(kBX, kBY);
+    const double alpha = visc * dt;
+    const double beta = 1.0 + 4.0 * alpha;  // src/fluid.cpp:145
+    const double gamma = -1.0 / dt;         // src/fluid.cpp:218
+    hipLaunchKernelGGL(k_t_advect, grid, block, 0, s, P, d[0], w, h, dt, (double)w, (double)h,
+                       1.0 / w, 1.0 / h, (w & (w - 1)) == 0, (h & (h - 1)) == 0);
+    for (int k = 1; k <= kd; ++k)
+        hipLaunchKernelGGL(k_t_diffuse, grid, block, 0, s, d[(k - 1) & 1], d[k & 1], w, h, alpha,
+                           beta);
+    hipLaunchKernelGGL(k_t_divergence, grid, block, 0, s, d[kd & 1], P, Q, b, p[0], w, h, gamma,
+                       kd % 2 == 0);
+    for (int k = 1; k <= kp; ++k)
+        hipLaunchKernelGGL(k_t_pressure, grid, block, 0, s, p[(k - 1) % 3], p[k % 3], b, w, h);
+    hipLaunchKernelGGL(k_t_finalize, grid, block, 0, s, P, Q, d[kd & 1], d[(kd + 1) & 1], p[0],
+                       p[1], p[2], b, w, h, dt, kd, kp);
+    return check_launch("FP64 twin step");
+}
+
+// ==================== FP32 precision form (temporally blocked) ===============
+constexpr int kRowB2 = kMaxK + 1;      // rows 0..kMaxK: sum r(p^i)^2; this row: sum b^2
+constexpr int kNRows = kMaxK + 2;
+constexpr int kMaxLaunch = kMaxK + 2;  // pressure launches: chunks of >= 1 sweep, the fix-up
+
+// Device-side state of one precision step (cleared by k_pctl_clear).
+struct PCtl {
+    unsigned count[kMaxLaunch];  // workgroups finished, per pressure launch
+    int done;                    // 1: vp and tmp hold the result
+    int stop;                    // first converged iterate a fold found, else -1
+    int kp;                      // pressure sweeps of the result
+    int pad;
+    double total[kNRows];        // folded sums
+};
+
+struct PreciseArgs {
+    float *P, *Q;               // vp, tmp: the outputs
+    float *pl[3];               // pressure iterate planes; pl[0] holds the start
+    const float *bpl;           // the divergence b
+    const float2 *E;            // the diffused velocity the outputs keep
+    double *part;               // per-tile FP64 sums, part[row * ntiles + tile]
+    PCtl *ctl;
+    int w, h, tiles_x, ntiles;
+    float dt;
+    int K, C, nchunks, launch;  // sweeps, sweeps per launch, chunks, this launch
+    double tol2;
+};
+
+__global__ void k_pctl_clear(PCtl *c) {
+    for (int k = threadIdx.x; k < kMaxLaunch; k += blockDim.x) c->count[k] = 0u;
+    for (int k = threadIdx.x; k < kNRows; k += blockDim.x) c->total[k] = 0.0;
+    if (threadIdx.x == 0) {
+        c->done = 0;
+        c->stop = -1;
+        c->kp = 0;
     }
 }
 
-template <typename T, bool COMP>
-int precise_step(T *P, T *Q, int w, int h, T dt, T visc, int kd, int kp, double tol, void *ws,
-                 hipStream_t s, Ctl *ctl) {
-    using T2 = typename V2<T>::type;
-    const size_t N = (size_t)w * h;
-    T2 *d[2] = {static_cast<T2 *>(ws), static_cast<T2 *>(ws) + N};
-    T *b = reinterpret_cast<T *>(d[1] + N);
-    T *p[3] = {b + N, b + 2 * N, b + 3 * N};
-    const dim3 grid((w + kBX - 1) / kBX, (h + kBY * kRows - 1) / (kBY * kRows)),
-        block(kBX, kBY);
-    const T alpha = visc * dt;
-    const T beta = (T)(1.0 + 4.0 * (double)alpha);  // src/fluid.cpp:145
-    const T gamma = (T)(-1.0 / (double)dt);         // src/fluid.cpp:218
-    const double tol2 = tol * tol;
-    if (COMP) hipLaunchKernelGGL(k_ctl_clear, dim3(kSums), dim3(256), 0, s, ctl);
-    hipLaunchKernelGGL((k_p_advect<T>), grid, block, 0, s, P, d[0], w, h, dt, (T)w, (T)h,
-                       (T)1 / (T)w, (T)1 / (T)h, (w & (w - 1)) == 0, (h & (h - 1)) == 0);
-    for (int k = 1; k <= kd; ++k)
-        hipLaunchKernelGGL((k_p_diffuse<T, COMP>), grid, block, 0, s, d[(k - 1) & 1], d[k & 1], w,
-                           h, alpha, beta);
-    hipLaunchKernelGGL((k_p_divergence<T, COMP>), grid, block, 0, s, d[kd & 1], P, Q, b, p[0], w,
-                       h, gamma, kd % 2 == 0, ctl);
-    for (int k = 1; k <= kp; ++k)
-        hipLaunchKernelGGL((k_p_pressure<T, COMP>), grid, block, 0, s, p[(k - 1) % 3], p[k % 3], b,
-                           w, h, k, tol2, ctl);
-    hipLaunchKernelGGL((k_p_finalize<T, COMP>), grid, block, 0, s, P, Q, d[kd & 1],
-                       d[(kd + 1) & 1], p[0], p[1], p[2], b, w, h, dt, kd, kp, tol2, ctl);
-    return check_launch("precision step");
+// Sum over the 64 lanes of a wave by a fixed butterfly: every lane ends
+// with the same value, independent of scheduling.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
 }
 
-// Workspace: two iterate planes of T2, b and three pressure planes of T,
-// then (FP32 only) the control block and its folded totals, each 256-byte
-// aligned.
-constexpr size_t kCtlBytes = (sizeof(Ctl) + 255) / 256 * 256;
+template <int H, bool LAST, int TS>
+__global__ __launch_bounds__((Geo<H, TS>::NT)) void k_diffuse_precise(DiffuseArgs a) {
+    diffuse_tile<H, LAST, TS, true>(a, xcd_tile(blockIdx.x, a.ntiles));
+}
 
-template <typename T>
+// Up to C = H - 1 compensated pressure sweeps of one tile, the residual of
+// every iterate they read, and (on the launch that ends the solve) the
+// gradient subtraction and the outputs.  See the file comment for the
+// launch sequence.
+template <int H, int TS>
+__global__ __launch_bounds__((Geo<H, TS>::NT)) void k_pressure_precise(PreciseArgs a) {
+    using G = Geo<H, TS>;
+    constexpr int S = G::S, RX = G::RX, RY = G::RY, PL = G::RYP * RX;
+    constexpr int C = H - 1;      // most sweeps of one launch
+    constexpr int NW = G::NT / 64;
+    constexpr int NSLOT = C + 2;  // rows one launch adds: C residuals, r(p^K), b^2
+    __shared__ float plane[2 * PL];
+    __shared__ double wsum[NSLOT][NW];
+    __shared__ double rowsum[NSLOT];
+    __shared__ int last_group;
+
+    // ---- what this launch does: every workgroup reads the same state -------
+    const int done = a.ctl->done, stop = a.ctl->stop;
+    if (done) return;
+    bool finish, fin;  // finish: re-run to the converged iterate; fin: write the outputs
+    int src, b0, ns;   // input plane, its iterate number, sweeps
+    if (stop >= 0) {
+        // The fold of launch k = launch - 1 (iterates kC .. kC + C - 1) found
+        // iterate `stop`: re-run from launch k's input, or from launch k-1's
+        // when `stop` is launch k's input itself (stop >= 1, so k >= 1 then).
+        const int k = a.launch - 1, kb = k * a.C;
+        finish = fin = true;
+        if (stop > kb) {
+            src = k % 3;
+            b0 = kb;
+        } else {
+            src = (k + 2) % 3;
+            b0 = kb - a.C;
+        }
+        ns = stop - b0;
+    } else if (a.launch < a.nchunks) {
+        finish = false;
+        src = a.launch % 3;
+        b0 = a.launch * a.C;
+        ns = min(a.C, a.K - b0);
+        fin = a.launch == a.nchunks - 1;
+    } else {
+        return;  // the fix-up launch, with nothing to fix
+    }
+
+    const int tile = xcd_tile(blockIdx.x, a.ntiles);
+    const Strip<G> st(threadIdx.x, tile, a.tiles_x, a.w, a.h);
+    const float *__restrict__ in = a.pl[src];
+    float p[S], rhs[S];
+    double b2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+        const int g = st.row_ok(k) ? st.gj[k] * a.w + st.gi : 0;
+        p[k] = in[g];
+        rhs[k] = a.bpl[g];
+    }
+    if (a.launch == 0) {
+#pragma unroll
+        for (int k = 0; k < S; ++k) b2 += st.mine(k, a.w, a.h) ? (double)rhs[k] * rhs[k] : 0.0;
+    }
+    float2 e[S];
+    if (fin) {
+#pragma unroll
+        for (int k = 0; k < S; ++k) e[k] = a.E[st.mine(k, a.w, a.h) ? st.cell(k, a.w) : 0];
+    }
+
+    // Sweeps (src/fluid.cpp:240-257), p' = RN(s + e) / 4 with (s, e) the
+    // compensated L + R + T + B + b; the residual of the iterate read,
+    // r = (s - 4p) + e in FP64, on the tile's own cells.
+    double rr[C];
+#pragma unroll
+    for (int m = 0; m < C; ++m) rr[m] = 0.0;
+    float pp[S];  // the iterate before the last sweep
+#pragma unroll
+    for (int s = 1; s <= C; ++s) {
+        if (s > ns) break;
+        float *B = plane + (s & 1) * PL;
+#pragma unroll
+        for (int k = 0; k < S; ++k)
+            if (st.active) B[(st.r0 + k) * RX + st.x] = p[k];
+        __syncthreads();
+        if (!st.active) continue;
+        const float top = B[(st.r0 > 0 ? st.r0 - 1 : 0) * RX + st.x];
+        const float bot = B[(st.r0 + S < RY ? st.r0 + S : RY - 1) * RX + st.x];
+        const float *Bl = B + st.r0 * RX + st.xl, *Br = B + st.r0 * RX + st.xr;
+        float prev = top;
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            const float cur = p[k];
+            pp[k] = cur;
+            const float below = k == S - 1 ? bot : p[k + 1];
+            float sv, ev;
+            pressure_sum_comp(Bl[k * RX], Br[k * RX], prev, below, rhs[k], sv, ev);
+            p[k] = (sv + ev) * 0.25f;
+            const double r = ((double)sv - 4.0 * (double)cur) + (double)ev;
+            rr[s - 1] += st.mine(k, a.w, a.h) ? r * r : 0.0;
+            prev = cur;
+        }
+    }
+
+    double rK = 0.0;
+    if (!fin) {
+        float *__restrict__ out = a.pl[(a.launch + 1) % 3];
+#pragma unroll
+        for (int k = 0; k < S; ++k)
+            if (st.mine(k, a.w, a.h)) out[st.cell(k, a.w)] = p[k];
+    } else {
+        // Gradient subtraction (src/fluid.cpp:273-295) and the outputs; the
+        // plane written here was last read in sweep ns - 1.
+        float *B = plane + ((ns + 1) & 1) * PL;
+#pragma unroll
+        for (int k = 0; k < S; ++k)
+            if (st.active) B[(st.r0 + k) * RX + st.x] = p[k];
+        __syncthreads();
+        float4 *__restrict__ P4 = reinterpret_cast<float4 *>(a.P);
+        float4 *__restrict__ Q4 = reinterpret_cast<float4 *>(a.Q);
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            if (!st.mine(k, a.w, a.h)) continue;
+            const int o = (st.r0 + k) * RX + st.x;
+            const float pL = B[o - 1], pR = B[o + 1], pT = B[o - RX], pB = B[o + RX];
+            const float gradX = (pR - pL) * a.dt / 2.0f;
+            const float gradY = (pB - pT) * a.dt / 2.0f;
+            const int cell = st.cell(k, a.w);
+            P4[cell] = make_float4(e[k].x - gradX, e[k].y - gradY, pp[k], rhs[k]);
+            Q4[cell] = make_float4(e[k].x, e[k].y, p[k], rhs[k]);
+            // Residual of p^K (the solve ran every sweep): FP64 from the floats.
+            const double r = (double)rhs[k] +
+                             (((double)pL + (double)pR) + ((double)pT + (double)pB)) -
+                             4.0 * (double)p[k];
+            rK += r * r;
+        }
+    }
+
+    // ---- this workgroup's FP64 sums into its slots -------------------------
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (!finish) {
+        double v[NSLOT];
+#pragma unroll
+        for (int m = 0; m < C; ++m) v[m] = rr[m];
+        v[C] = rK;
+        v[C + 1] = b2;
+#pragma unroll
+        for (int m = 0; m < NSLOT; ++m) {
+            const double t = wave_sum(v[m]);
+            if (lane == 0) wsum[m][wv] = t;
+        }
+        __syncthreads();
+        if ((int)threadIdx.x < NSLOT) {
+            double t = 0.0;
+            for (int w = 0; w < NW; ++w) t += wsum[threadIdx.x][w];
+            rowsum[threadIdx.x] = t;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int m = 0; m < ns; ++m) a.part[(size_t)(b0 + m) * a.ntiles + tile] = rowsum[m];
+            if (fin) a.part[(size_t)a.K * a.ntiles + tile] = rowsum[C];
+            if (a.launch == 0) a.part[(size_t)kRowB2 * a.ntiles + tile] = rowsum[C + 1];
+        }
+    }
+    // ---- the last workgroup of the launch folds and decides -----------------
+    if (threadIdx.x == 0) {
+        __threadfence();  // release: this workgroup's slots before its count
+        const unsigned n = atomicAdd(&a.ctl->count[a.launch], 1u);
+        last_group = n == (unsigned)a.ntiles - 1u;
+    }
+    __syncthreads();
+    if (!last_group) return;
+    __threadfence();  // acquire: every workgroup's slots
+    auto fold = [&](int row) {
+        double t = 0.0;
+        for (int i = threadIdx.x; i < a.ntiles; i += G::NT)
+            t += __hip_atomic_load(&a.part[(size_t)row * a.ntiles + i], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        t = wave_sum(t);
+        if (lane == 0) wsum[0][wv] = t;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double tot = 0.0;
+            for (int w = 0; w < NW; ++w) tot += wsum[0][w];
+            a.ctl->total[row] = tot;
+        }
+        __syncthreads();
+    };
+    if (finish) {
+        if (threadIdx.x == 0) {
+            a.ctl->kp = stop;
+            a.ctl->done = 1;
+        }
+        return;
+    }
+    for (int m = 0; m < ns; ++m) fold(b0 + m);
+    if (fin) fold(a.K);
+    if (a.launch == 0) fold(kRowB2);
+    if (threadIdx.x == 0) {
+        const double lim = a.tol2 * a.ctl->total[kRowB2];
+        int found = -1;
+        for (int i = b0 > 1 ? b0 : 1; i < b0 + ns; ++i)
+            if (a.ctl->total[i] <= lim) {
+                found = i;
+                break;
+            }
+        if (found >= 0) {
+            a.ctl->stop = found;
+        } else if (fin) {
+            a.ctl->kp = a.K;
+            a.ctl->done = 1;
+        }
+    }
+}
+
+// ---- host dispatch --------------------------------------------------------------
+using DiffuseFn = void (*)(DiffuseArgs);
+using PressureFn = void (*)(PreciseArgs);
+
+// Diffusion launches: chunks of C = 4 or 8 sweeps (halo C), the last one
+// of 1..C sweeps (halo 2..C+1).
+template <int TS>
+DiffuseFn diffuse_precise_ts(int H, bool last) {
+    if (!last) return H == 4 ? k_diffuse_precise<4, false, TS>
+                             : H == 8 ? k_diffuse_precise<8, false, TS> : nullptr;
+    switch (H) {
+#define PFS_CASE(h) \
+    case h:         \
+        return k_diffuse_precise<h, true, TS>;
+        PFS_CASE(2) PFS_CASE(3) PFS_CASE(4) PFS_CASE(5) PFS_CASE(6) PFS_CASE(7) PFS_CASE(8)
+        PFS_CASE(9)
+#undef PFS_CASE
+    }
+    return nullptr;
+}
+
+DiffuseFn diffuse_precise_for(int H, bool last, int ts) {
+    return ts == 16 ? diffuse_precise_ts<16>(H, last)
+                    : ts == 32 ? diffuse_precise_ts<32>(H, last) : diffuse_precise_ts<64>(H, last);
+}
+
+template <int H>
+PressureFn pressure_precise_h(int ts) {
+    return ts == 16 ? k_pressure_precise<H, 16>
+                    : ts == 32 ? k_pressure_precise<H, 32> : k_pressure_precise<H, 64>;
+}
+
+PressureFn pressure_precise_for(int C, int ts) {
+    return C == 4 ? pressure_precise_h<5>(ts) : pressure_precise_h<9>(ts);
+}
+
+template <int TS>
+int threads_precise_ts(int H) {
+    switch (H) {
+#define PFS_CASE(h) \
+    case h:         \
+        return Geo<h, TS>::NT;
+        PFS_CASE(2) PFS_CASE(3) PFS_CASE(4) PFS_CASE(5) PFS_CASE(6) PFS_CASE(7) PFS_CASE(8)
+        PFS_CASE(9)
+#undef PFS_CASE
+    }
+    return 0;
+}
+
+int threads_precise(int H, int ts) {
+    return ts == 16 ? threads_precise_ts<16>(H)
+                    : ts == 32 ? threads_precise_ts<32>(H) : threads_precise_ts<64>(H);
+}
+
+// Sweeps per launch (4 or 8) and tile side (16, 32, 64; 0: the plain
+// step's rule), process-wide.
+std::atomic<int> g_precise_chunk{8};
+std::atomic<int> g_precise_tile{0};
+
+// Workspace: the plain fused step's 8 planes (pressure start, b, two float2
+// diffusion planes, two pressure planes), the control block, and the
+// per-tile sums for the most tiles any side gives.
+constexpr size_t kPCtlBytes = (sizeof(PCtl) + 255) / 256 * 256;
 size_t precise_ws_bytes(int w, int h) {
     const size_t N = (size_t)w * h;
-    return N * (2 * sizeof(typename V2<T>::type) + 4 * sizeof(T)) + kCtlBytes +
-           sizeof(CtlTotals) + 256;
+    const size_t planes = ((8 * N * sizeof(float)) + 255) / 256 * 256;
+    return planes + kPCtlBytes + (size_t)kNRows * (size_t)tile_count(w, h, 16) * sizeof(double) +
+           256;
+}
+
+int precise_step(float *P, float *Q, int w, int h, float dt, float visc, int K, double tol,
+                 void *ws, hipStream_t s, PCtl **ctl_out) {
+    const int C = g_precise_chunk.load();
+    const int forced = g_precise_tile.load();
+    const int ts = forced ? forced : tile_for_counts(tile_count(w, h, 64), tile_count(w, h, 32));
+    const size_t N = (size_t)w * h;
+    char *base = reinterpret_cast<char *>(((uintptr_t)ws + 255) & ~(uintptr_t)255);
+    float *wsf = reinterpret_cast<float *>(base);
+    PCtl *ctl = reinterpret_cast<PCtl *>(base + ((8 * N * sizeof(float)) + 255) / 256 * 256);
+    double *part = reinterpret_cast<double *>(reinterpret_cast<char *>(ctl) + kPCtlBytes);
+    *ctl_out = ctl;
+
+    hipLaunchKernelGGL(k_pctl_clear, dim3(1), dim3(256), 0, s, ctl);
+    DiffuseArgs d = diffuse_args(P, Q, w, h, dt, visc, K, wsf, ts);
+    Chunk plan[kMaxK];
+    const int nd = chunk_plan(K, plan, kMaxK, C);
+    for (int c = 0; c < nd; ++c) {
+        diffuse_chunk(d, c, plan[c], wsf);
+        hipLaunchKernelGGL(diffuse_precise_for(plan[c].H, plan[c].last, ts), dim3(d.ntiles),
+                           dim3(threads_precise(plan[c].H, ts)), 0, s, d);
+        PFS_TRY(check_launch("precise diffuse"));
+    }
+    PreciseArgs pa{};
+    pa.P = P;
+    pa.Q = Q;
+    pa.pl[0] = wsf;  // d.pw: the pressure start
+    pa.pl[1] = wsf + 6 * N;
+    pa.pl[2] = wsf + 7 * N;
+    pa.bpl = wsf + N;
+    pa.E = reinterpret_cast<const float2 *>(wsf + 2 * N + 2 * N * ((nd - 1) & 1));
+    pa.part = part;
+    pa.ctl = ctl;
+    pa.w = w;
+    pa.h = h;
+    pa.tiles_x = d.tiles_x;
+    pa.ntiles = d.ntiles;
+    pa.dt = dt;
+    pa.K = K;
+    pa.C = C;
+    pa.nchunks = (K + C - 1) / C;
+    pa.tol2 = tol * tol;
+    const PressureFn pf = pressure_precise_for(C, ts);
+    const int nt = threads_precise(C + 1, ts);
+    for (int l = 0; l <= pa.nchunks; ++l) {  // the chunks, then the fix-up
+        pa.launch = l;
+        hipLaunchKernelGGL(pf, dim3(pa.ntiles), dim3(nt), 0, s, pa);
+        PFS_TRY(check_launch("precise pressure"));
+    }
+    return PFS_OK;
 }
 
 }  // namespace
@@ -383,7 +605,17 @@ extern "C" {
 
 size_t pfs_precise_workspace_bytes(int vx, int vy, int fp64) {
     if (vx < 1 || vy < 1) return 0;
-    return fp64 ? precise_ws_bytes<double>(vx, vy) : precise_ws_bytes<float>(vx, vy);
+    return fp64 ? twin_ws_bytes(vx, vy) : precise_ws_bytes(vx, vy);
+}
+
+int pfs_set_precise_schedule(int sweeps_per_launch, int tile) {
+    if (sweeps_per_launch != 0 && sweeps_per_launch != 4 && sweeps_per_launch != 8)
+        return set_error(PFS_ERR_INVALID, "sweeps per launch must be 0 (default), 4 or 8");
+    if (tile != 0 && !valid_tile_side(tile))
+        return set_error(PFS_ERR_INVALID, "tile side must be 0 (automatic), 16, 32 or 64");
+    g_precise_chunk = sweeps_per_launch ? sweeps_per_launch : 8;
+    g_precise_tile = tile;
+    return PFS_OK;
 }
 
 int pfs_simulate_fluid_step_precise(pfs_field *vp, pfs_field *tmp, float dt, float viscosity,
@@ -395,28 +627,26 @@ int pfs_simulate_fluid_step_precise(pfs_field *vp, pfs_field *tmp, float dt, flo
     if (vp->x != tmp->x || vp->y != tmp->y || vp->z != 4 || tmp->z != 4)
         return set_error(PFS_ERR_INVALID, "vp and tmp must both be W x H x 4");
     if (vp->data == tmp->data) return set_error(PFS_ERR_INVALID, "vp and tmp alias");
+    if ((uintptr_t)vp->data % 16 || (uintptr_t)tmp->data % 16)
+        return set_error(PFS_ERR_UNSUPPORTED, "vp and tmp must be 16-byte aligned");
     if (max_iters < 1 || max_iters > kMaxK)
         return set_error(PFS_ERR_INVALID, "max_iters must be in [1, PFS_PRECISE_MAX_ITERS]");
-    if (!workspace || workspace_bytes < precise_ws_bytes<float>(vp->x, vp->y))
+    if (!(tol >= 0.0)) return set_error(PFS_ERR_INVALID, "tol must be >= 0");
+    if (!workspace || workspace_bytes < precise_ws_bytes(vp->x, vp->y))
         return set_error(PFS_ERR_INVALID, "workspace smaller than pfs_precise_workspace_bytes()");
     hipStream_t s = (hipStream_t)stream;
-    const size_t N = (size_t)vp->x * vp->y;
-    char *ctl_at = static_cast<char *>(workspace) + N * (2 * sizeof(float2) + 4 * sizeof(float));
-    Ctl *ctl = reinterpret_cast<Ctl *>(((uintptr_t)ctl_at + 255) & ~(uintptr_t)255);
-    PFS_TRY((precise_step<float, true>(vp->data, tmp->data, vp->x, vp->y, dt, viscosity, max_iters,
-                                       max_iters, tol, workspace, s, ctl)));
+    PCtl *ctl = nullptr;
+    PFS_TRY(precise_step(vp->data, tmp->data, vp->x, vp->y, dt, viscosity, max_iters, tol,
+                         workspace, s, &ctl));
     if (report) {
-        CtlTotals *tot = reinterpret_cast<CtlTotals *>(reinterpret_cast<char *>(ctl) + kCtlBytes);
-        hipLaunchKernelGGL(k_ctl_fold, dim3(1), dim3(128), 0, s, ctl, tot);
-        PFS_TRY(check_launch("precision report"));
-        CtlTotals h;
-        PFS_HIP_TRY(hipMemcpyAsync(&h, tot, sizeof(h), hipMemcpyDeviceToHost, s));
+        PCtl h;
+        PFS_HIP_TRY(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, s));
         PFS_HIP_TRY(hipStreamSynchronize(s));
         std::memset(report, 0, sizeof(*report));
         report->iters_used = h.kp;
-        report->b_norm = std::sqrt(h.sum[kSumB2]);
-        report->residual_norm = std::sqrt(h.sum[kSumFinal]);
-        for (int k = 0; k <= kMaxK; ++k) report->residual_history[k] = std::sqrt(h.sum[k]);
+        report->b_norm = std::sqrt(h.total[kRowB2]);
+        report->residual_norm = std::sqrt(h.total[h.kp]);
+        for (int k = 0; k <= kMaxK; ++k) report->residual_history[k] = std::sqrt(h.total[k]);
     }
     return PFS_OK;
 }
@@ -430,11 +660,10 @@ int pfs_simulate_fluid_step_f64(double *vp, double *tmp, int vx, int vy, double 
     if (iters_diffuse < 1 || iters_pressure < 1 || iters_pressure > iters_diffuse ||
         iters_diffuse > kMaxK)
         return set_error(PFS_ERR_INVALID, "need 1 <= iters_pressure <= iters_diffuse <= max");
-    if (!workspace || workspace_bytes < precise_ws_bytes<double>(vx, vy))
+    if (!workspace || workspace_bytes < twin_ws_bytes(vx, vy))
         return set_error(PFS_ERR_INVALID, "workspace smaller than pfs_precise_workspace_bytes()");
-    return precise_step<double, false>(vp, tmp, vx, vy, dt, viscosity, iters_diffuse,
-                                       iters_pressure, 0.0, workspace, (hipStream_t)stream,
-                                       nullptr);
+    return twin_step(vp, tmp, vx, vy, dt, viscosity, iters_diffuse, iters_pressure, workspace,
+                     (hipStream_t)stream);
 }
 
 }  // extern "C"

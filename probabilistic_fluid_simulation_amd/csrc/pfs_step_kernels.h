@@ -111,6 +111,50 @@ struct Strip {
     __device__ int cell(int k, int w) const { return (y0 + r0 + k) * w + x0 + x; }
 };
 
+// Error-free transformation of a sum (Knuth's TwoSum): s = RN(a + b) and
+// a + b = s + e exactly.
+__device__ __forceinline__ float two_sum_err(float a, float b, float s) {
+    const float bb = s - a;
+    return (a - (s - bb)) + (b - bb);
+}
+
+// alpha*(L + R + T + B) + C, the numerator of the diffusion sweep
+// (src/fluid.cpp:19-21,175-182: alpha*L + alpha*R + alpha*T + alpha*B + 1*C,
+// the same real number), evaluated compensated: the neighbour sum by TwoSum
+// with its errors kept, the product by alpha with its exact FMA error
+// (TwoProduct) plus alpha times the kept errors, the centre by TwoSum, and
+// one final rounding of the value carried in about twice FP32 precision
+// (Ogita, Rump & Oishi 2005, Sum2/Dot2).
+__device__ __forceinline__ float diffuse_num_comp(float l, float r, float t, float b, float c,
+                                                  float alpha) {
+    const float s1 = l + r;
+    float e = two_sum_err(l, r, s1);
+    const float s2 = s1 + t;
+    e += two_sum_err(s1, t, s2);
+    const float s3 = s2 + b;
+    e += two_sum_err(s2, b, s3);
+    const float p = alpha * s3;
+    float ep = fmaf(alpha, s3, -p);
+    ep = fmaf(alpha, e, ep);
+    const float y = p + c;
+    return y + (two_sum_err(p, c, y) + ep);
+}
+
+// L + R + T + B + rhs of the pressure sweep (src/fluid.cpp:249-255), as a
+// compensated pair: s = the running FP32 sum, e = its accumulated TwoSum
+// errors, s + e the sum in about twice FP32 precision.
+__device__ __forceinline__ void pressure_sum_comp(float l, float r, float t, float b, float rhs,
+                                                  float &s, float &e) {
+    const float s1 = l + r;
+    e = two_sum_err(l, r, s1);
+    const float s2 = s1 + t;
+    e += two_sum_err(s1, t, s2);
+    const float s3 = s2 + b;
+    e += two_sum_err(s2, b, s3);
+    s = s3 + rhs;
+    e += two_sum_err(s3, rhs, s);
+}
+
 // Region coordinate of global index g (0 <= g < n) on an axis whose staged
 // region starts at global coordinate o (periodic), or `lim` when g + 1 is
 // not staged right after it.
@@ -121,7 +165,10 @@ __device__ __forceinline__ int staged(int g, int o, int n, int lim) {
     return r < lim - 1 ? r : lim;
 }
 
-template <int H, bool LAST, int TS = kTile>
+// COMP (the precision form): the numerators are compensated
+// (diffuse_num_comp); the strip publishes its raw values instead of
+// alpha-scaled ones.  COMP = false is the reference's arithmetic.
+template <int H, bool LAST, int TS = kTile, bool COMP = false>
 __device__ __forceinline__ void diffuse_tile(const DiffuseArgs &a, int tile) {
     using G = Geo<H, TS>;
     constexpr int S = G::S, RX = G::RX, RY = G::RY, PL = G::RYP * RX;
@@ -200,11 +247,11 @@ __device__ __forceinline__ void diffuse_tile(const DiffuseArgs &a, int tile) {
     const bool fast = a.fast_div != 0;
     for (int s = 1; s <= NS; ++s) {
         float2 *B = plane + ((s + 1) & 1) * PL;
-        float au[S], av[S];
+        float au[S], av[S];  // what the strip publishes: alpha-scaled, or raw (COMP)
 #pragma unroll
         for (int k = 0; k < S; ++k) {
-            au[k] = a.alpha * u[k];
-            av[k] = a.alpha * v[k];
+            au[k] = COMP ? u[k] : a.alpha * u[k];
+            av[k] = COMP ? v[k] : a.alpha * v[k];
             if (st.active) B[(st.r0 + k) * RX + st.x] = make_float2(au[k], av[k]);
         }
         __syncthreads();
@@ -225,8 +272,13 @@ __device__ __forceinline__ void diffuse_tile(const DiffuseArgs &a, int tile) {
             const float tu = k == 0 ? top.x : au[k - 1], tv = k == 0 ? top.y : av[k - 1];
             const float bu = k == S - 1 ? bot.x : au[k + 1];
             const float bv = k == S - 1 ? bot.y : av[k + 1];
-            nu[k] = L.x + R.x + tu + bu + 1.0f * u[k];
-            nv[k] = L.y + R.y + tv + bv + 1.0f * v[k];
+            if constexpr (COMP) {
+                nu[k] = diffuse_num_comp(L.x, R.x, tu, bu, u[k], a.alpha);
+                nv[k] = diffuse_num_comp(L.y, R.y, tv, bv, v[k], a.alpha);
+            } else {
+                nu[k] = L.x + R.x + tu + bu + 1.0f * u[k];
+                nv[k] = L.y + R.y + tv + bv + 1.0f * v[k];
+            }
             lo = fminf(lo, fminf(fabsf(nu[k]), fabsf(nv[k])));
             hi = fmaxf(hi, fmaxf(fabsf(nu[k]), fabsf(nv[k])));
         }
@@ -411,10 +463,10 @@ struct Chunk {
     int ns, H;
     bool first, last;
 };
-inline int chunk_plan(int iters, Chunk *out, int max_chunks) {
+inline int chunk_plan(int iters, Chunk *out, int max_chunks, int chunk = kMaxChunk) {
     int n = 0;
     for (int done = 0; done < iters && n < max_chunks; ++n) {
-        const int ns = (iters - done) < kMaxChunk ? (iters - done) : kMaxChunk;
+        const int ns = (iters - done) < chunk ? (iters - done) : chunk;
         out[n].ns = ns;
         out[n].first = done == 0;
         out[n].last = done + ns == iters;

@@ -219,6 +219,12 @@ def simulate_fluid_step_precise(vp: VPField, tmp: VPField, dt: float, viscosity:
             "residual_history": list(rep.residual_history)[:max(rep.iters_used + 2, 1)]}
 
 
+def set_precise_schedule(sweeps_per_launch: int = 0, tile: int = 0) -> None:
+    """The precision form's sweeps per launch (4 or 8; 0: default) and tile
+    side (16, 32, 64; 0: automatic); results do not depend on either."""
+    _lib.call("pfs_set_precise_schedule", sweeps_per_launch, tile)
+
+
 def simulate_fluid_step_f64(vp: torch.Tensor, tmp: torch.Tensor, dt: float, viscosity: float,
                             iters_diffuse: int, iters_pressure: int, stream=None) -> None:
     """FP64 twin of the step on (H, W, 4) float64 device tensors: plain double

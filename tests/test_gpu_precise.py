@@ -138,3 +138,112 @@ def test_batched_and_served_scenes_within_1e5_of_fp64():
             assert float(err.max()) <= 1e-5, (j.job, j.fw, float(err.max()))
             for _ in range(20):
                 fluid.fluid_timestep(vp, tmp, im, itmp, j.dt, j.visc, j.iters)
+
+
+# ------------------------------------------------- schedule, exit and determinism
+SCHEDULES = [(4, 16), (4, 32), (4, 64), (8, 16), (8, 32), (8, 64)]
+
+
+def _run_precise(s, tol, sched, max_iters=16):
+    fluid.set_precise_schedule(*sched)
+    try:
+        v, t = VPField.from_numpy(s.vp), VPField.from_numpy(s.tmp)
+        rep = fluid.simulate_fluid_step_precise(v, t, s.dt, s.visc, max_iters, tol)
+        return rep, v.numpy(), t.numpy()
+    finally:
+        fluid.set_precise_schedule(0, 0)
+
+
+def _bits(hist):
+    return np.asarray(hist, dtype=np.float64).tobytes()
+
+
+def test_precise_runs_are_bitwise_reproducible():
+    """The FP64 sums are folded in a fixed order: two runs of the same step
+    give the same residual history, exit and outputs, bit for bit."""
+    s = scenes.make_scene(520, 390, 4, 4, kind="tg", seed=12, dt=1 / 60, visc=0.1, iters=16)
+    a = _run_precise(s, 1e-4, (8, 0))
+    b = _run_precise(s, 1e-4, (8, 0))
+    assert a[0]["iters_used"] == b[0]["iters_used"]
+    assert _bits(a[0]["residual_history"]) == _bits(b[0]["residual_history"])
+    assert a[0]["b_norm"] == b[0]["b_norm"] and a[0]["residual_norm"] == b[0]["residual_norm"]
+    assert a[1].tobytes() == b[1].tobytes() and a[2].tobytes() == b[2].tobytes()
+
+
+@pytest.fixture(scope="module")
+def exit_scene():
+    """A 200x150 Taylor-Green scene and the relative residuals of its 16
+    pressure iterates (tol = 0: every sweep runs)."""
+    s = scenes.make_scene(200, 150, 4, 4, kind="tg", seed=9, dt=1 / 60, visc=0.1, iters=16)
+    assert fluid.get_step_tile(200, 150, 16) == 16  # the default schedule is (8, 16)
+    rep, _, _ = _run_precise(s, 0.0, (8, 0))
+    assert rep["iters_used"] == 16
+    rel = [r / rep["b_norm"] for r in rep["residual_history"][:17]]
+    return s, rep, rel
+
+
+@pytest.mark.parametrize("target", [1, 3, 4, 5, 7, 8, 9, 12, 13, 15, 16])
+def test_precise_exit_is_the_same_for_every_schedule(exit_scene, target):
+    """A tolerance just above the relative residual of iterate `target`
+    (and below every earlier one) stops the solve there, whatever the
+    launches' chunk size and tile side: at a chunk's first iterate (a re-run
+    of the chunk before), inside a chunk, in the last chunk (the fix-up
+    launch), and not at all (16).  The outputs are bit-identical across
+    schedules, the residual history equals the full solve's up to the exit,
+    and the result is within FP32 rounding of the FP64 twin run with that
+    many pressure sweeps.  (The FP64 sums are partitioned by tile: across
+    tile sides and chunk sizes the history agrees to FP64 rounding; with one
+    schedule it is the same bits on every run.)"""
+    s, full, rel = exit_scene
+    if target < 16:
+        earlier = min(rel[1:target]) if target > 1 else np.inf
+        if not rel[target] * (1 + 1e-6) < earlier:
+            pytest.skip("the residual does not fall to a new low at this iterate")
+        tol = rel[target] * (1 + 1e-9)
+        want = target
+    else:
+        tol = min(rel[1:16]) * 0.5
+        want = 16
+    outs = [_run_precise(s, tol, sched) for sched in SCHEDULES]
+    rep0, v0, t0 = outs[0]
+    assert rep0["iters_used"] == want
+    for (rep, v, t), sched in zip(outs[1:], SCHEDULES[1:]):
+        assert rep["iters_used"] == want, sched
+        assert v.tobytes() == v0.tobytes() and t.tobytes() == t0.tobytes(), sched
+        np.testing.assert_allclose(rep["residual_history"][:want + 1],
+                                   rep0["residual_history"][:want + 1], rtol=1e-12)
+    # The full solve ran with the default schedule here, (8, 16): same bits.
+    assert (_bits(outs[SCHEDULES.index((8, 16))][0]["residual_history"][:want + 1])
+            == _bits(full["residual_history"][:want + 1]))
+    assert rep0["residual_norm"] == rep0["residual_history"][want]
+    v64, t64 = _dev64(s.vp), _dev64(s.tmp)
+    fluid.simulate_fluid_step_f64(v64, t64, s.dt, s.visc, 16, want)
+    torch.cuda.synchronize()
+    # Errors relative to the largest magnitude: velocity 1e-5 (the bound the
+    # FP32 forms are held to), pressure 1e-4 (it inherits the cancellation
+    # in the divergence of FP32-rounded velocities, scaled by 1/dt).
+    ref_t, ref_v = t64.cpu().numpy(), v64.cpu().numpy()
+    err_p = np.abs(t0[..., 2] - ref_t[..., 2]).max() / np.abs(ref_t[..., 2]).max()
+    err_v = (np.abs(v0[..., :2] - ref_v[..., :2]).max(axis=(0, 1))
+             / np.abs(ref_v[..., :2]).max(axis=(0, 1)))
+    assert err_p < 1e-4 and np.all(err_v < 1e-5), (err_p, err_v)
+    r, bn = _residual(t0)
+    assert rep0["residual_norm"] == pytest.approx(r, rel=1e-5, abs=1e-9)
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 5, 8, 9, 17, 64])
+def test_precise_sweep_counts(k):
+    """Every sweep count, up to PFS_PRECISE_MAX_ITERS, against the FP64 twin
+    (tol = 0: all k sweeps), for both chunk sizes, identical between them."""
+    s = scenes.make_scene(96, 80, 4, 4, kind="tg", seed=3, dt=1 / 60, visc=0.1, iters=k)
+    a = _run_precise(s, 0.0, (4, 0), max_iters=k)
+    b = _run_precise(s, 0.0, (8, 0), max_iters=k)
+    assert a[0]["iters_used"] == b[0]["iters_used"] == k
+    assert a[1].tobytes() == b[1].tobytes() and a[2].tobytes() == b[2].tobytes()
+    v64, t64 = _dev64(s.vp), _dev64(s.tmp)
+    fluid.simulate_fluid_step_f64(v64, t64, s.dt, s.visc, k, k)
+    torch.cuda.synchronize()
+    ref = v64.cpu().numpy()
+    scale = np.abs(ref[..., :2]).max(axis=(0, 1))
+    err = np.abs(a[1][..., :2] - ref[..., :2]).max(axis=(0, 1)) / scale
+    assert np.all(err < 1e-5), err

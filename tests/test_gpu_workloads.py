@@ -49,6 +49,13 @@ def test_precision_form_at_2048():
         r, bn = _residual64(tmp.numpy())
         assert rep["residual_norm"] == pytest.approx(r, rel=1e-6), step
         assert rep["b_norm"] == pytest.approx(bn, rel=1e-6), step
+        # The same step again from the same state: the same bits (the FP64
+        # sums are folded in a fixed order).
+        again_v, again_t = VPField(before[0].clone()), VPField(before[1].clone())
+        rep2 = fluid.simulate_fluid_step_precise(again_v, again_t, s.dt, s.visc, 16, 1e-4)
+        assert (np.asarray(rep2["residual_history"]).tobytes()
+                == np.asarray(rep["residual_history"]).tobytes()), step
+        assert torch.equal(again_v.data, vp.data) and torch.equal(again_t.data, tmp.data), step
         v64, t64 = before[0].double(), before[1].double()
         fluid.simulate_fluid_step_f64(v64, t64, s.dt, s.visc, 16, 16)
         ref = v64[..., :2]
