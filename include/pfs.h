@@ -120,6 +120,21 @@ int pfs_advect_color_step(pfs_field *image, pfs_field *itmp, const pfs_field *vp
 int pfs_fluid_timestep(pfs_field *vp, pfs_field *tmp, pfs_field *image, pfs_field *itmp, float dt,
                        float viscosity, int iters, void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): pfs_fluid_timestep, same arguments and results, with
+ * the fused step's workspace in a device array the library holds per GPU
+ * (PFS_NOALLOC_SCRATCH_BYTES, from the first call on a GPU until the process
+ * ends) instead of a block it allocates.  Calls no hipMalloc or hipFree.  A
+ * step whose workspace exceeds the array (16 B per field cell for
+ * iters <= 8, 32 B beyond: up to 2M cells, or 1M past 8 sweeps) is rejected
+ * with PFS_ERR_UNSUPPORTED before any launch.  Every call on one GPU uses
+ * the same array: calls that overlap on one GPU (other streams or threads,
+ * not ordered) race. */
+#define PFS_NOALLOC_SCRATCH_BYTES (32u << 20)
+int pfs_fluid_timestep_noalloc(pfs_field *vp, pfs_field *tmp, pfs_field *image, pfs_field *itmp,
+                               float dt, float viscosity, int iters, void *stream);
+
 /* includes/fluid.hpp:107.  Device-flavour step on raw device pointers (the
  * signature src/main.cpp:222 calls, plus iters and stream).  Same results
  * as pfs_simulate_fluid_step; *vp and *tmp are left un-swapped. */

@@ -73,6 +73,7 @@ SIGNATURES = {
     "pfs_advect_color_step": [_P, _P, _P, _F, _V],
     "pfs_fluid_timestep": [_P, _P, _P, _P, _F, _F, _I, _V],
     "pfs_fluid_run": [_P, _P, _P, _P, _F, _F, _I, _I, _V],
+    "pfs_fluid_timestep_noalloc": [_P, _P, _P, _P, _F, _F, _I, _V],
     "pfs_simulate_fluid_step_dev": [_FPP, _FPP, _F, _F, _I, _I, _I, _I, _V],
     "pfs_advect_color_step_dev": [_FPP, _FPP, _FPP, _F, _I, _I, _I, _I, _I, _I, _V],
     "pfs_set_step_path": [_I],

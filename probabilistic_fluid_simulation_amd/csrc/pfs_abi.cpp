@@ -264,9 +264,12 @@ static bool fused_ok(const pfs_field *vp, const pfs_field *tmp, int iters) {
 // simulate_fluid_step, optionally followed by advect_color_step on
 // (image, itmp) as one timestep of src/main.cpp:222-225.  `ev` (4 events)
 // requests per-kernel timing of the fused path.
+//
+// `ws_given` (with `ws_cap` bytes) replaces the stream's library-allocated
+// workspace; a fused step that needs more is rejected before any launch.
 static int timestep(pfs_field *vp, pfs_field *tmp, pfs_field *image, pfs_field *itmp, float dt,
                     float viscosity, int iters, void *stream, hipEvent_t *ev, int reps = 1,
-                    int tile = 0) {
+                    int tile = 0, float *ws_given = nullptr, size_t ws_cap = 0) {
     PFS_TRY(check_step_fields(vp, tmp, iters));
     if (image) {
         PFS_TRY(check_field(image, "image", 1));
@@ -286,8 +289,18 @@ static int timestep(pfs_field *vp, pfs_field *tmp, pfs_field *image, pfs_field *
     // (i, j): same size, 4 channels, float4-aligned.
     const bool dye_fused = image && image->x == vp->x && image->y == vp->y && image->z == 4 &&
                            aligned16(image->data) && aligned16(itmp->data);
-    float *ws = nullptr;
-    PFS_TRY(get_workspace(fused_workspace_bytes(vp->x, vp->y), (hipStream_t)stream, &ws));
+    float *ws = ws_given;
+    if (ws_given) {
+        const size_t need = scene_ws_floats(vp->x, vp->y, iters) * sizeof(float);
+        if (need > ws_cap)
+            return set_error(PFS_ERR_UNSUPPORTED,
+                             "a " + std::to_string(vp->x) + "x" + std::to_string(vp->y) +
+                                 " field with iters = " + std::to_string(iters) + " needs " +
+                                 std::to_string(need) + " bytes of step scratch; the " +
+                                 "no-allocation entry point holds " + std::to_string(ws_cap));
+    } else {
+        PFS_TRY(get_workspace(fused_workspace_bytes(vp->x, vp->y), (hipStream_t)stream, &ws));
+    }
     StepArgs a{vp->data, tmp->data, dye_fused ? image->data : nullptr,
                dye_fused ? itmp->data : nullptr, vp->x, vp->y, dt, viscosity, iters,
                tile ? tile : step_tile(vp->x, vp->y, iters, dye_fused)};
@@ -433,6 +446,47 @@ int pfs_fluid_timestep(pfs_field *vp, pfs_field *tmp, pfs_field *image, pfs_fiel
                        float viscosity, int iters, void *stream) {
     if (!image || !itmp) return set_error(PFS_ERR_INVALID, "image or itmp is NULL");
     return timestep(vp, tmp, image, itmp, dt, viscosity, iters, stream, nullptr);
+}
+
+// ---- the timestep that allocates nothing --------------------------------------
+// The fused step's workspace planes (pressure start, divergence, E and, past
+// 8 sweeps, the iterate planes: scene_ws_floats) in a file-scope device
+// array instead of a hipMalloc'd block.  Each GPU the library runs on holds
+// its own copy, at its own address, until the process ends.  Host code uses
+// the address hipGetSymbolAddress gives for the current device, never the
+// array's name (a host-side stand-in).  Nothing in it needs clearing: every
+// plane is written by one kernel before the next reads it.
+static __device__ __align__(256) float scratch[PFS_NOALLOC_SCRATCH_BYTES / sizeof(float)];
+
+static std::mutex g_scratch_mutex;
+static std::map<int, float *> g_scratch;  // device -> address of its `scratch`
+
+static int scratch_here(float **out) {
+    int dev = 0;
+    PFS_HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    auto it = g_scratch.find(dev);
+    if (it == g_scratch.end()) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return set_error(PFS_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                              hipGetErrorString(e));
+        }
+        it = g_scratch.emplace(dev, static_cast<float *>(p)).first;
+    }
+    *out = it->second;
+    return PFS_OK;
+}
+
+int pfs_fluid_timestep_noalloc(pfs_field *vp, pfs_field *tmp, pfs_field *image, pfs_field *itmp,
+                               float dt, float viscosity, int iters, void *stream) {
+    if (!image || !itmp) return set_error(PFS_ERR_INVALID, "image or itmp is NULL");
+    float *ws = nullptr;
+    PFS_TRY(scratch_here(&ws));
+    return timestep(vp, tmp, image, itmp, dt, viscosity, iters, stream, nullptr, 1, 0, ws,
+                    PFS_NOALLOC_SCRATCH_BYTES);
 }
 
 int pfs_advect_color_step(pfs_field *image, pfs_field *itmp, const pfs_field *vp, float dt,

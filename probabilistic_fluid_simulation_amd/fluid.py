@@ -174,6 +174,17 @@ def fluid_timestep(vp: VPField, tmp: VPField, image: VPField, itmp: VPField, dt:
     b.sync_back()
 
 
+def fluid_timestep_noalloc(vp: VPField, tmp: VPField, image: VPField, itmp: VPField, dt: float,
+                           viscosity: float, iters: int = NUM_JACOBI_ITERS, stream=None) -> None:
+    """fluid_timestep with the step's workspace in the library's per-GPU
+    device array (pfs_fluid_timestep_noalloc): allocates nothing; fields
+    whose workspace exceeds PFS_NOALLOC_SCRATCH_BYTES are rejected."""
+    b = _Bound(vp, tmp, image, itmp)
+    _lib.call("pfs_fluid_timestep_noalloc", b.ptr(0), b.ptr(1), b.ptr(2), b.ptr(3), dt,
+              viscosity, iters, _stream(stream))
+    b.sync_back()
+
+
 def fluid_run(vp: VPField, tmp: VPField, image: VPField, itmp: VPField, dt: float,
               viscosity: float, iters: int, nsteps: int, stream=None) -> None:
     """nsteps timesteps in one call (fluid_timestep nsteps times, identical
