@@ -281,10 +281,13 @@ int pfs_simulate_fluid_step_precise(pfs_field *vp, pfs_field *tmp, float dt, flo
                                     int max_iters, double tol, void *workspace,
                                     size_t workspace_bytes, void *stream,
                                     pfs_precise_report *report);
-/* Sweeps per launch of the precision form's temporally blocked kernels (4
- * or 8; 0: the default, 8) and their tile side (16, 32, 64; 0: the plain
- * step's rule), process-wide.  Results do not depend on either. */
-int pfs_set_precise_schedule(int sweeps_per_launch, int tile);
+/* Schedule of the precision form's temporally blocked kernels,
+ * process-wide: sweeps per launch (4 or 8; 0: the default, 8), tile side
+ * (16, 32, 64; 0: the plain step's rule) and threads of the pressure
+ * workgroups at tile side 64 (512, two per CU, or 1024; 0: the default,
+ * 512).  The outputs do not depend on it; the FP64 residual sums are
+ * partitioned by it (the same bits on every run of one schedule). */
+int pfs_set_precise_schedule(int sweeps_per_launch, int tile, int pressure_threads);
 /* FP64 twin: the same step and contract on double [vy][vx][4] device buffers
  * with plain arithmetic and given sweep counts (1 <= iters_pressure <=
  * iters_diffuse); with equal counts it is the reference's algorithm in

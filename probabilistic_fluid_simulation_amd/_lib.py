@@ -103,7 +103,7 @@ SIGNATURES = {
     "pfs_batch_autotune": [_V, _I, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float),
                            _I],
     "pfs_precise_workspace_bytes": [_I, _I, _I],
-    "pfs_set_precise_schedule": [_I, _I],
+    "pfs_set_precise_schedule": [_I, _I, _I],
     "pfs_simulate_fluid_step_precise": [_P, _P, _F, _F, _I, ctypes.c_double, _V, ctypes.c_size_t,
                                         _V, ctypes.POINTER(PreciseReport)],
     "pfs_simulate_fluid_step_f64": [_V, _V, _I, _I, ctypes.c_double, ctypes.c_double, _I, _I, _V,

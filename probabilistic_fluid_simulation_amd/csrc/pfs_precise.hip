@@ -5,9 +5,9 @@
 // step structure (src/fluid.cpp:298-310) and changes three things:
 //
 //   - the 5-term stencil sums of both Jacobi sweeps (src/fluid.cpp:19-21)
-//     are compensated: sums by TwoSum with their errors carried, the product
-//     by alpha with its exact FMA error (Ogita, Rump & Oishi 2005,
-//     Sum2/Dot2), so each numerator is rounded once from about twice FP32
+//     are compensated: Kahan's compensated summation of the neighbours,
+//     the product by alpha with its exact FMA error, the diffusion's centre
+//     term by TwoSum, so each numerator is rounded once from about twice FP32
 //     precision before the division;
 //   - every pressure sweep also accumulates, in FP64, the residual of its
 //     input iterate, r = b + pL + pR + pT + pB - 4p (zero at the fixed point
@@ -19,10 +19,10 @@
 // Kernels (temporally blocked like the plain step, pfs_step.hip, on the
 // same register-strip tiles, pfs_step_kernels.h):
 //
-//   k_pctl_clear          control block of the step
 //   k_diffuse_precise     advect + C diffusion sweeps per launch (compensated),
 //                         the last launch also the divergence and the
-//                         pressure start (diffuse_tile<..., COMP = true>)
+//                         pressure start (diffuse_tile<..., COMP = true>);
+//                         the first also clears the step's control block
 //   k_pressure_precise    C pressure sweeps per launch from an iterate plane,
 //                         with the FP64 residual of every iterate the launch
 //                         reads on the tile's own cells; launch `launch`
@@ -218,7 +218,7 @@ constexpr int kRowB2 = kMaxK + 1;      // rows 0..kMaxK: sum r(p^i)^2; this row:
 constexpr int kNRows = kMaxK + 2;
 constexpr int kMaxLaunch = kMaxK + 2;  // pressure launches: chunks of >= 1 sweep, the fix-up
 
-// Device-side state of one precision step (cleared by k_pctl_clear).
+// Device-side state of one precision step (cleared by its first diffusion launch).
 struct PCtl {
     unsigned count[kMaxLaunch];  // workgroups finished, per pressure launch
     int done;                    // 1: vp and tmp hold the result
@@ -241,16 +241,6 @@ struct PreciseArgs {
     double tol2;
 };
 
-__global__ void k_pctl_clear(PCtl *c) {
-    for (int k = threadIdx.x; k < kMaxLaunch; k += blockDim.x) c->count[k] = 0u;
-    for (int k = threadIdx.x; k < kNRows; k += blockDim.x) c->total[k] = 0.0;
-    if (threadIdx.x == 0) {
-        c->done = 0;
-        c->stop = -1;
-        c->kp = 0;
-    }
-}
-
 // Sum over the 64 lanes of a wave by a fixed butterfly: every lane ends
 // with the same value, independent of scheduling.
 __device__ __forceinline__ double wave_sum(double v) {
@@ -259,8 +249,42 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// ctl non-null (the step's first launch): workgroup 0 also clears the
+// control block the pressure launches use.
+// x plus the value DPP control CTRL moves into this lane (0 where it moves
+// none), on the two 32-bit halves of a double.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_add(double x) {
+    const int lo = __double2loint(x), hi = __double2hiint(x);
+    const int mlo = __builtin_amdgcn_update_dpp(0, lo, CTRL, ROWS, 0xf, false);
+    const int mhi = __builtin_amdgcn_update_dpp(0, hi, CTRL, ROWS, 0xf, false);
+    return x + __hiloint2double(mhi, mlo);
+}
+
+// Inclusive scan of x over the wave in a fixed pattern (row_shr 1, 2, 4, 8,
+// then row_bcast 15 and 31): lane 63 ends with the wave's sum.  DPP moves
+// only, no LDS round trip.
+__device__ __forceinline__ double wave_scan_dpp(double x) {
+    x = dpp_add<0x111, 0xf>(x);  // row_shr:1
+    x = dpp_add<0x112, 0xf>(x);  // row_shr:2
+    x = dpp_add<0x114, 0xf>(x);  // row_shr:4
+    x = dpp_add<0x118, 0xf>(x);  // row_shr:8
+    x = dpp_add<0x142, 0xa>(x);  // row_bcast:15
+    x = dpp_add<0x143, 0xc>(x);  // row_bcast:31
+    return x;
+}
+
 template <int H, bool LAST, int TS>
-__global__ __launch_bounds__((Geo<H, TS>::NT)) void k_diffuse_precise(DiffuseArgs a) {
+__global__ __launch_bounds__((Geo<H, TS>::NT)) void k_diffuse_precise(DiffuseArgs a, PCtl *ctl) {
+    if (ctl && blockIdx.x == 0) {
+        for (int k = threadIdx.x; k < kMaxLaunch; k += blockDim.x) ctl->count[k] = 0u;
+        for (int k = threadIdx.x; k < kNRows; k += blockDim.x) ctl->total[k] = 0.0;
+        if (threadIdx.x == 0) {
+            ctl->done = 0;
+            ctl->stop = -1;
+            ctl->kp = 0;
+        }
+    }
     diffuse_tile<H, LAST, TS, true>(a, xcd_tile(blockIdx.x, a.ntiles));
 }
 
@@ -268,9 +292,11 @@ __global__ __launch_bounds__((Geo<H, TS>::NT)) void k_diffuse_precise(DiffuseArg
 // every iterate they read, and (on the launch that ends the solve) the
 // gradient subtraction and the outputs.  See the file comment for the
 // launch sequence.
-template <int H, int TS>
-__global__ __launch_bounds__((Geo<H, TS>::NT)) void k_pressure_precise(PreciseArgs a) {
-    using G = Geo<H, TS>;
+// NTMAX = 512: at most 128 VGPRs, so that two workgroups share a CU.
+template <int H, int TS, int NTMAX>
+__global__ __launch_bounds__((Geo<H, TS, NTMAX>::NT))
+__attribute__((amdgpu_waves_per_eu(NTMAX == 512 ? 4 : 1))) void k_pressure_precise(PreciseArgs a) {
+    using G = Geo<H, TS, NTMAX>;
     constexpr int S = G::S, RX = G::RX, RY = G::RY, PL = G::RYP * RX;
     constexpr int C = H - 1;      // most sweeps of one launch
     constexpr int NW = G::NT / 64;
@@ -324,19 +350,21 @@ __global__ __launch_bounds__((Geo<H, TS>::NT)) void k_pressure_precise(PreciseAr
 #pragma unroll
         for (int k = 0; k < S; ++k) b2 += st.mine(k, a.w, a.h) ? (double)rhs[k] * rhs[k] : 0.0;
     }
+    // E, which only the outputs need: loaded before the sweeps so that its
+    // latency hides behind them, except in 512-thread workgroups, whose
+    // longer strips leave no registers for it.
+    constexpr bool kEarlyE = NTMAX == 1024;
     float2 e[S];
-    if (fin) {
+    if (kEarlyE && fin) {
 #pragma unroll
         for (int k = 0; k < S; ++k) e[k] = a.E[st.mine(k, a.w, a.h) ? st.cell(k, a.w) : 0];
     }
 
-    // Sweeps (src/fluid.cpp:240-257), p' = RN(s + e) / 4 with (s, e) the
+    // Sweeps (src/fluid.cpp:240-257), p' = RN(s - k) / 4 with (s, k) the
     // compensated L + R + T + B + b; the residual of the iterate read,
-    // r = (s - 4p) + e in FP64, on the tile's own cells.
-    double rr[C];
-#pragma unroll
-    for (int m = 0; m < C; ++m) rr[m] = 0.0;
-    float pp[S];  // the iterate before the last sweep
+    // r = (s - 4p) - k in FP64, on the tile's own cells, summed over the
+    // thread's rows, then over the wave (DPP) into wsum[s - 1][wave].
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
     for (int s = 1; s <= C; ++s) {
         if (s > ns) break;
@@ -345,23 +373,27 @@ __global__ __launch_bounds__((Geo<H, TS>::NT)) void k_pressure_precise(PreciseAr
         for (int k = 0; k < S; ++k)
             if (st.active) B[(st.r0 + k) * RX + st.x] = p[k];
         __syncthreads();
-        if (!st.active) continue;
-        const float top = B[(st.r0 > 0 ? st.r0 - 1 : 0) * RX + st.x];
-        const float bot = B[(st.r0 + S < RY ? st.r0 + S : RY - 1) * RX + st.x];
-        const float *Bl = B + st.r0 * RX + st.xl, *Br = B + st.r0 * RX + st.xr;
-        float prev = top;
+        double racc = 0.0;
+        if (st.active) {
+            const float top = B[(st.r0 > 0 ? st.r0 - 1 : 0) * RX + st.x];
+            const float bot = B[(st.r0 + S < RY ? st.r0 + S : RY - 1) * RX + st.x];
+            const float *Bl = B + st.r0 * RX + st.xl, *Br = B + st.r0 * RX + st.xr;
+            float prev = top;
 #pragma unroll
-        for (int k = 0; k < S; ++k) {
-            const float cur = p[k];
-            pp[k] = cur;
-            const float below = k == S - 1 ? bot : p[k + 1];
-            float sv, ev;
-            pressure_sum_comp(Bl[k * RX], Br[k * RX], prev, below, rhs[k], sv, ev);
-            p[k] = (sv + ev) * 0.25f;
-            const double r = ((double)sv - 4.0 * (double)cur) + (double)ev;
-            rr[s - 1] += st.mine(k, a.w, a.h) ? r * r : 0.0;
-            prev = cur;
+            for (int k = 0; k < S; ++k) {
+                const float cur = p[k];
+                const float below = k == S - 1 ? bot : p[k + 1];
+                float sv, kv;
+                pressure_sum_comp(Bl[k * RX], Br[k * RX], prev, below, rhs[k], sv, kv);
+                p[k] = (sv - kv) * 0.25f;
+                const double r = fma(-4.0, (double)cur, (double)sv) - (double)kv;
+                const double rm = st.mine(k, a.w, a.h) ? r : 0.0;
+                racc = fma(rm, rm, racc);
+                prev = cur;
+            }
         }
+        racc = wave_scan_dpp(racc);
+        if (lane == 63) wsum[s - 1][wv] = racc;
     }
 
     double rK = 0.0;
@@ -372,8 +404,20 @@ __global__ __launch_bounds__((Geo<H, TS>::NT)) void k_pressure_precise(PreciseAr
             if (st.mine(k, a.w, a.h)) out[st.cell(k, a.w)] = p[k];
     } else {
         // Gradient subtraction (src/fluid.cpp:273-295) and the outputs; the
-        // plane written here was last read in sweep ns - 1.
+        // plane written here was last read in sweep ns - 1.  The iterate
+        // before the last sweep is what this thread published in it (its own
+        // slots of plane ns & 1, which nothing has overwritten).
+        float pp[S];
+        {
+            const float *Bl = plane + (ns & 1) * PL;
+#pragma unroll
+            for (int k = 0; k < S; ++k) pp[k] = st.active ? Bl[(st.r0 + k) * RX + st.x] : 0.0f;
+        }
         float *B = plane + ((ns + 1) & 1) * PL;
+        if (!kEarlyE) {
+#pragma unroll
+            for (int k = 0; k < S; ++k) e[k] = a.E[st.mine(k, a.w, a.h) ? st.cell(k, a.w) : 0];
+        }
 #pragma unroll
         for (int k = 0; k < S; ++k)
             if (st.active) B[(st.r0 + k) * RX + st.x] = p[k];
@@ -391,25 +435,19 @@ __global__ __launch_bounds__((Geo<H, TS>::NT)) void k_pressure_precise(PreciseAr
             P4[cell] = make_float4(e[k].x - gradX, e[k].y - gradY, pp[k], rhs[k]);
             Q4[cell] = make_float4(e[k].x, e[k].y, p[k], rhs[k]);
             // Residual of p^K (the solve ran every sweep): FP64 from the floats.
-            const double r = (double)rhs[k] +
-                             (((double)pL + (double)pR) + ((double)pT + (double)pB)) -
-                             4.0 * (double)p[k];
-            rK += r * r;
+            const double r = fma(-4.0, (double)p[k],
+                                 (double)rhs[k] + (((double)pL + (double)pR) +
+                                                   ((double)pT + (double)pB)));
+            rK = fma(r, r, rK);
         }
     }
 
     // ---- this workgroup's FP64 sums into its slots -------------------------
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (!finish) {
-        double v[NSLOT];
-#pragma unroll
-        for (int m = 0; m < C; ++m) v[m] = rr[m];
-        v[C] = rK;
-        v[C + 1] = b2;
-#pragma unroll
-        for (int m = 0; m < NSLOT; ++m) {
-            const double t = wave_sum(v[m]);
-            if (lane == 0) wsum[m][wv] = t;
+        const double tk = wave_sum(rK), tb = wave_sum(b2);
+        if (lane == 0) {
+            wsum[C][wv] = tk;
+            wsum[C + 1][wv] = tb;
         }
         __syncthreads();
         if ((int)threadIdx.x < NSLOT) {
@@ -433,21 +471,6 @@ __global__ __launch_bounds__((Geo<H, TS>::NT)) void k_pressure_precise(PreciseAr
     __syncthreads();
     if (!last_group) return;
     __threadfence();  // acquire: every workgroup's slots
-    auto fold = [&](int row) {
-        double t = 0.0;
-        for (int i = threadIdx.x; i < a.ntiles; i += G::NT)
-            t += __hip_atomic_load(&a.part[(size_t)row * a.ntiles + i], __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-        t = wave_sum(t);
-        if (lane == 0) wsum[0][wv] = t;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double tot = 0.0;
-            for (int w = 0; w < NW; ++w) tot += wsum[0][w];
-            a.ctl->total[row] = tot;
-        }
-        __syncthreads();
-    };
     if (finish) {
         if (threadIdx.x == 0) {
             a.ctl->kp = stop;
@@ -455,15 +478,38 @@ __global__ __launch_bounds__((Geo<H, TS>::NT)) void k_pressure_precise(PreciseAr
         }
         return;
     }
-    for (int m = 0; m < ns; ++m) fold(b0 + m);
-    if (fin) fold(a.K);
-    if (a.launch == 0) fold(kRowB2);
+    // Slot m of this launch is row b0 + m (m < ns), then row K (fin), then
+    // b^2 (launch 0).  Wave w folds slots w, w + NW, ... over the tiles:
+    // lane j sums tiles j, j + 64, ... with four accumulators in a fixed
+    // pattern, then the butterfly; every total is the same on every run.
+    const int nrow = ns + (fin ? 1 : 0) + (a.launch == 0 ? 1 : 0);
+    for (int m = wv; m < nrow; m += NW) {
+        const int row = m < ns ? b0 + m : (fin && m == ns ? a.K : kRowB2);
+        const double *src = a.part + (size_t)row * a.ntiles;
+        double t[4] = {0.0, 0.0, 0.0, 0.0};
+        int i = lane;
+        for (; i + 192 < a.ntiles; i += 256) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                t[q] += __hip_atomic_load(src + i + 64 * q, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+        }
+        for (int q = 0; i < a.ntiles; i += 64, ++q)
+            t[q] += __hip_atomic_load(src + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const double tot = wave_sum((t[0] + t[1]) + (t[2] + t[3]));
+        if (lane == 0) {
+            a.ctl->total[row] = tot;
+            rowsum[m] = tot;
+        }
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        const double lim = a.tol2 * a.ctl->total[kRowB2];
+        const double b2 = a.launch == 0 ? rowsum[nrow - 1] : a.ctl->total[kRowB2];
+        const double lim = a.tol2 * b2;
         int found = -1;
-        for (int i = b0 > 1 ? b0 : 1; i < b0 + ns; ++i)
-            if (a.ctl->total[i] <= lim) {
-                found = i;
+        for (int m = b0 >= 1 ? 0 : 1; m < ns; ++m)
+            if (rowsum[m] <= lim) {
+                found = b0 + m;
                 break;
             }
         if (found >= 0) {
@@ -476,7 +522,7 @@ __global__ __launch_bounds__((Geo<H, TS>::NT)) void k_pressure_precise(PreciseAr
 }
 
 // ---- host dispatch --------------------------------------------------------------
-using DiffuseFn = void (*)(DiffuseArgs);
+using DiffuseFn = void (*)(DiffuseArgs, PCtl *);
 using PressureFn = void (*)(PreciseArgs);
 
 // Diffusion launches: chunks of C = 4 or 8 sweeps (halo C), the last one
@@ -502,13 +548,14 @@ DiffuseFn diffuse_precise_for(int H, bool last, int ts) {
 }
 
 template <int H>
-PressureFn pressure_precise_h(int ts) {
-    return ts == 16 ? k_pressure_precise<H, 16>
-                    : ts == 32 ? k_pressure_precise<H, 32> : k_pressure_precise<H, 64>;
+PressureFn pressure_precise_h(int ts, bool half) {
+    if (ts == 64 && half) return k_pressure_precise<H, 64, 512>;
+    return ts == 16 ? k_pressure_precise<H, 16, 1024>
+                    : ts == 32 ? k_pressure_precise<H, 32, 1024> : k_pressure_precise<H, 64, 1024>;
 }
 
-PressureFn pressure_precise_for(int C, int ts) {
-    return C == 4 ? pressure_precise_h<5>(ts) : pressure_precise_h<9>(ts);
+PressureFn pressure_precise_for(int C, int ts, bool half) {
+    return C == 4 ? pressure_precise_h<5>(ts, half) : pressure_precise_h<9>(ts, half);
 }
 
 template <int TS>
@@ -529,10 +576,19 @@ int threads_precise(int H, int ts) {
                     : ts == 32 ? threads_precise_ts<32>(H) : threads_precise_ts<64>(H);
 }
 
+int threads_pressure_precise(int C, int ts, bool half) {
+    if (ts == 64 && half) return C == 4 ? Geo<5, 64, 512>::NT : Geo<9, 64, 512>::NT;
+    return threads_precise(C + 1, ts);
+}
+
 // Sweeps per launch (4 or 8) and tile side (16, 32, 64; 0: the plain
 // step's rule), process-wide.
 std::atomic<int> g_precise_chunk{8};
 std::atomic<int> g_precise_tile{0};
+// 1: 512-thread pressure workgroups at tile side 64 (two per CU: one's
+// loads and barriers overlap the other's sweeps; 0.37 against 0.42 ms per
+// 2048^2 step with 1024).
+std::atomic<int> g_precise_half{1};
 
 // Workspace: the plain fused step's 8 planes (pressure start, b, two float2
 // diffusion planes, two pressure planes), the control block, and the
@@ -557,14 +613,13 @@ int precise_step(float *P, float *Q, int w, int h, float dt, float visc, int K, 
     double *part = reinterpret_cast<double *>(reinterpret_cast<char *>(ctl) + kPCtlBytes);
     *ctl_out = ctl;
 
-    hipLaunchKernelGGL(k_pctl_clear, dim3(1), dim3(256), 0, s, ctl);
     DiffuseArgs d = diffuse_args(P, Q, w, h, dt, visc, K, wsf, ts);
     Chunk plan[kMaxK];
     const int nd = chunk_plan(K, plan, kMaxK, C);
     for (int c = 0; c < nd; ++c) {
         diffuse_chunk(d, c, plan[c], wsf);
         hipLaunchKernelGGL(diffuse_precise_for(plan[c].H, plan[c].last, ts), dim3(d.ntiles),
-                           dim3(threads_precise(plan[c].H, ts)), 0, s, d);
+                           dim3(threads_precise(plan[c].H, ts)), 0, s, d, c == 0 ? ctl : nullptr);
         PFS_TRY(check_launch("precise diffuse"));
     }
     PreciseArgs pa{};
@@ -586,8 +641,9 @@ int precise_step(float *P, float *Q, int w, int h, float dt, float visc, int K, 
     pa.C = C;
     pa.nchunks = (K + C - 1) / C;
     pa.tol2 = tol * tol;
-    const PressureFn pf = pressure_precise_for(C, ts);
-    const int nt = threads_precise(C + 1, ts);
+    const bool half = g_precise_half.load() != 0;
+    const PressureFn pf = pressure_precise_for(C, ts, half);
+    const int nt = threads_pressure_precise(C, ts, half);
     for (int l = 0; l <= pa.nchunks; ++l) {  // the chunks, then the fix-up
         pa.launch = l;
         hipLaunchKernelGGL(pf, dim3(pa.ntiles), dim3(nt), 0, s, pa);
@@ -608,13 +664,16 @@ size_t pfs_precise_workspace_bytes(int vx, int vy, int fp64) {
     return fp64 ? twin_ws_bytes(vx, vy) : precise_ws_bytes(vx, vy);
 }
 
-int pfs_set_precise_schedule(int sweeps_per_launch, int tile) {
+int pfs_set_precise_schedule(int sweeps_per_launch, int tile, int pressure_threads) {
     if (sweeps_per_launch != 0 && sweeps_per_launch != 4 && sweeps_per_launch != 8)
         return set_error(PFS_ERR_INVALID, "sweeps per launch must be 0 (default), 4 or 8");
     if (tile != 0 && !valid_tile_side(tile))
         return set_error(PFS_ERR_INVALID, "tile side must be 0 (automatic), 16, 32 or 64");
+    if (pressure_threads != 0 && pressure_threads != 512 && pressure_threads != 1024)
+        return set_error(PFS_ERR_INVALID, "pressure threads must be 0 (default), 512 or 1024");
     g_precise_chunk = sweeps_per_launch ? sweeps_per_launch : 8;
     g_precise_tile = tile;
+    g_precise_half = pressure_threads != 1024 ? 1 : 0;
     return PFS_OK;
 }
 

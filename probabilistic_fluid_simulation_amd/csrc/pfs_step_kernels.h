@@ -12,20 +12,22 @@ constexpr int kTile = 64;    // the largest tile side (the default)
 constexpr int kSegs = 12;    // row segments per staged column of a 64-wide tile
 constexpr int kMaxChunk = 8; // sweeps per launch
 
-template <int H_, int TS_ = kTile>
+template <int H_, int TS_ = kTile, int NTMAX = 1024>
 struct Geo {
     static constexpr int H = H_;
     static constexpr int TX = TS_, TY = TS_;
     static constexpr int RX = TX + 2 * H, RY = TY + 2 * H;
     // Rows per thread: a 64-wide tile keeps 12 segments per column; smaller
     // tiles take as many segments as 1024 threads allow, rows split evenly.
-    static constexpr int MAXSEG = TS_ == kTile ? kSegs : 1024 / RX;
+    // NTMAX < 1024: workgroups of at most NTMAX threads (longer strips), so
+    // that two fit a CU.
+    static constexpr int MAXSEG = (TS_ == kTile && NTMAX == 1024) ? kSegs : NTMAX / RX;
     static constexpr int S = (RY + MAXSEG - 1) / MAXSEG;
-    static constexpr int SEGS = TS_ == kTile ? kSegs : (RY + S - 1) / S;
+    static constexpr int SEGS = (TS_ == kTile && NTMAX == 1024) ? kSegs : (RY + S - 1) / S;
     static constexpr int RYP = SEGS * S;                // rows of an LDS plane (>= RY)
     static constexpr int NTH = RX * SEGS;               // threads owning cells
     static constexpr int NT = ((NTH + 63) / 64) * 64;   // whole wavefronts
-    static_assert(NTH <= 1024 && RYP >= RY, "tile geometry");
+    static_assert(NTH <= NTMAX && RYP >= RY, "tile geometry");
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
@@ -120,39 +122,47 @@ __device__ __forceinline__ float two_sum_err(float a, float b, float s) {
 
 // alpha*(L + R + T + B) + C, the numerator of the diffusion sweep
 // (src/fluid.cpp:19-21,175-182: alpha*L + alpha*R + alpha*T + alpha*B + 1*C,
-// the same real number), evaluated compensated: the neighbour sum by TwoSum
-// with its errors kept, the product by alpha with its exact FMA error
-// (TwoProduct) plus alpha times the kept errors, the centre by TwoSum, and
-// one final rounding of the value carried in about twice FP32 precision
-// (Ogita, Rump & Oishi 2005, Sum2/Dot2).
+// the same real number), evaluated compensated: the neighbour sum by Kahan's
+// compensated summation (running sum s, correction c, the sum being s - c;
+// Kahan 1965), its product by alpha with the exact FMA error (TwoProduct)
+// less alpha times c, the centre added by TwoSum (the two terms are not
+// ordered by magnitude), and one final rounding.
 __device__ __forceinline__ float diffuse_num_comp(float l, float r, float t, float b, float c,
                                                   float alpha) {
-    const float s1 = l + r;
-    float e = two_sum_err(l, r, s1);
-    const float s2 = s1 + t;
-    e += two_sum_err(s1, t, s2);
-    const float s3 = s2 + b;
-    e += two_sum_err(s2, b, s3);
-    const float p = alpha * s3;
-    float ep = fmaf(alpha, s3, -p);
-    ep = fmaf(alpha, e, ep);
-    const float y = p + c;
-    return y + (two_sum_err(p, c, y) + ep);
+    float s = l + r;
+    float k = (s - l) - r;
+    float y = t - k, u = s + y;
+    k = (u - s) - y;
+    s = u;
+    y = b - k;
+    u = s + y;
+    k = (u - s) - y;
+    s = u;
+    const float p = alpha * s;
+    float ep = fmaf(alpha, s, -p);
+    ep = fmaf(-alpha, k, ep);
+    const float q = p + c;
+    return q + (two_sum_err(p, c, q) + ep);
 }
 
-// L + R + T + B + rhs of the pressure sweep (src/fluid.cpp:249-255), as a
-// compensated pair: s = the running FP32 sum, e = its accumulated TwoSum
-// errors, s + e the sum in about twice FP32 precision.
+// L + R + T + B + rhs of the pressure sweep (src/fluid.cpp:249-255) by
+// Kahan's compensated summation: s the running FP32 sum, k its correction,
+// s - k the sum in about twice FP32 precision.
 __device__ __forceinline__ void pressure_sum_comp(float l, float r, float t, float b, float rhs,
-                                                  float &s, float &e) {
-    const float s1 = l + r;
-    e = two_sum_err(l, r, s1);
-    const float s2 = s1 + t;
-    e += two_sum_err(s1, t, s2);
-    const float s3 = s2 + b;
-    e += two_sum_err(s2, b, s3);
-    s = s3 + rhs;
-    e += two_sum_err(s3, rhs, s);
+                                                  float &s, float &k) {
+    s = l + r;
+    k = (s - l) - r;
+    float y = t - k, u = s + y;
+    k = (u - s) - y;
+    s = u;
+    y = b - k;
+    u = s + y;
+    k = (u - s) - y;
+    s = u;
+    y = rhs - k;
+    u = s + y;
+    k = (u - s) - y;
+    s = u;
 }
 
 // Region coordinate of global index g (0 <= g < n) on an axis whose staged

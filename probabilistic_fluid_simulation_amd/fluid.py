@@ -230,10 +230,12 @@ def simulate_fluid_step_precise(vp: VPField, tmp: VPField, dt: float, viscosity:
             "residual_history": list(rep.residual_history)[:max(rep.iters_used + 2, 1)]}
 
 
-def set_precise_schedule(sweeps_per_launch: int = 0, tile: int = 0) -> None:
-    """The precision form's sweeps per launch (4 or 8; 0: default) and tile
-    side (16, 32, 64; 0: automatic); results do not depend on either."""
-    _lib.call("pfs_set_precise_schedule", sweeps_per_launch, tile)
+def set_precise_schedule(sweeps_per_launch: int = 0, tile: int = 0,
+                         pressure_threads: int = 0) -> None:
+    """The precision form's sweeps per launch (4 or 8), tile side (16, 32,
+    64) and pressure workgroup threads at side 64 (512 or 1024); 0: the
+    default of each.  The outputs do not depend on the schedule."""
+    _lib.call("pfs_set_precise_schedule", sweeps_per_launch, tile, pressure_threads)
 
 
 def simulate_fluid_step_f64(vp: torch.Tensor, tmp: torch.Tensor, dt: float, viscosity: float,

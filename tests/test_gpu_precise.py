@@ -141,7 +141,8 @@ def test_batched_and_served_scenes_within_1e5_of_fp64():
 
 
 # ------------------------------------------------- schedule, exit and determinism
-SCHEDULES = [(4, 16), (4, 32), (4, 64), (8, 16), (8, 32), (8, 64)]
+SCHEDULES = [(4, 16, 0), (4, 32, 0), (4, 64, 512), (4, 64, 1024), (8, 16, 0), (8, 32, 0),
+             (8, 64, 512), (8, 64, 1024)]
 
 
 def _run_precise(s, tol, sched, max_iters=16):
@@ -151,7 +152,7 @@ def _run_precise(s, tol, sched, max_iters=16):
         rep = fluid.simulate_fluid_step_precise(v, t, s.dt, s.visc, max_iters, tol)
         return rep, v.numpy(), t.numpy()
     finally:
-        fluid.set_precise_schedule(0, 0)
+        fluid.set_precise_schedule(0, 0, 0)
 
 
 def _bits(hist):
@@ -162,8 +163,8 @@ def test_precise_runs_are_bitwise_reproducible():
     """The FP64 sums are folded in a fixed order: two runs of the same step
     give the same residual history, exit and outputs, bit for bit."""
     s = scenes.make_scene(520, 390, 4, 4, kind="tg", seed=12, dt=1 / 60, visc=0.1, iters=16)
-    a = _run_precise(s, 1e-4, (8, 0))
-    b = _run_precise(s, 1e-4, (8, 0))
+    a = _run_precise(s, 1e-4, (8, 0, 0))
+    b = _run_precise(s, 1e-4, (8, 0, 0))
     assert a[0]["iters_used"] == b[0]["iters_used"]
     assert _bits(a[0]["residual_history"]) == _bits(b[0]["residual_history"])
     assert a[0]["b_norm"] == b[0]["b_norm"] and a[0]["residual_norm"] == b[0]["residual_norm"]
@@ -176,7 +177,7 @@ def exit_scene():
     pressure iterates (tol = 0: every sweep runs)."""
     s = scenes.make_scene(200, 150, 4, 4, kind="tg", seed=9, dt=1 / 60, visc=0.1, iters=16)
     assert fluid.get_step_tile(200, 150, 16) == 16  # the default schedule is (8, 16)
-    rep, _, _ = _run_precise(s, 0.0, (8, 0))
+    rep, _, _ = _run_precise(s, 0.0, (8, 0, 0))
     assert rep["iters_used"] == 16
     rel = [r / rep["b_norm"] for r in rep["residual_history"][:17]]
     return s, rep, rel
@@ -213,7 +214,7 @@ def test_precise_exit_is_the_same_for_every_schedule(exit_scene, target):
         np.testing.assert_allclose(rep["residual_history"][:want + 1],
                                    rep0["residual_history"][:want + 1], rtol=1e-12)
     # The full solve ran with the default schedule here, (8, 16): same bits.
-    assert (_bits(outs[SCHEDULES.index((8, 16))][0]["residual_history"][:want + 1])
+    assert (_bits(outs[SCHEDULES.index((8, 16, 0))][0]["residual_history"][:want + 1])
             == _bits(full["residual_history"][:want + 1]))
     assert rep0["residual_norm"] == rep0["residual_history"][want]
     v64, t64 = _dev64(s.vp), _dev64(s.tmp)
@@ -236,8 +237,8 @@ def test_precise_sweep_counts(k):
     """Every sweep count, up to PFS_PRECISE_MAX_ITERS, against the FP64 twin
     (tol = 0: all k sweeps), for both chunk sizes, identical between them."""
     s = scenes.make_scene(96, 80, 4, 4, kind="tg", seed=3, dt=1 / 60, visc=0.1, iters=k)
-    a = _run_precise(s, 0.0, (4, 0), max_iters=k)
-    b = _run_precise(s, 0.0, (8, 0), max_iters=k)
+    a = _run_precise(s, 0.0, (4, 64, 512), max_iters=k)
+    b = _run_precise(s, 0.0, (8, 64, 1024), max_iters=k)
     assert a[0]["iters_used"] == b[0]["iters_used"] == k
     assert a[1].tobytes() == b[1].tobytes() and a[2].tobytes() == b[2].tobytes()
     v64, t64 = _dev64(s.vp), _dev64(s.tmp)
