@@ -854,11 +854,15 @@ def main() -> int:
         out["roofline"]["rocprof_frac"] = alg[dom] / (rp_us * 1e-6) / 1e9 / HBM_PEAK_GBS
         out["roofline"]["kernel_us_events"] = avg_ms[dom] * 1e3
     valu = pmc_figure("valu_insts_per_launch", dom)
+    dual = pmc_figure("valu_dual_issue_quads_per_launch", dom) or 0
     if valu is not None:
-        # The kernel's other bound: the time its VALU instructions take to
-        # issue, as a fraction of its measured duration.
-        out["roofline"]["valu_issue_frac"] = valu / VALU_WAVE_INSTS_PER_S / (avg_ms[dom] * 1e-3)
+        # The kernel's other bound: the time its VALU instructions keep the
+        # SIMDs busy (one quad-cycle per instruction, less the quad-cycles
+        # that issue two), as a fraction of its measured duration.
+        out["roofline"]["valu_issue_frac"] = ((valu - dual) / VALU_WAVE_INSTS_PER_S
+                                              / (avg_ms[dom] * 1e-3))
         out["roofline"]["valu_insts_per_launch"] = valu
+        out["roofline"]["valu_dual_issue_quads_per_launch"] = dual
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scenes.plain_1m(), args.cpu_steps)
         out["cpu_baseline"]["whole_run_check"] = whole_run_check(s, end_state,
