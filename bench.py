@@ -674,10 +674,11 @@ def served_capacity_leg(n_jobs: int, seed: int = 7) -> dict:
     0.1, seed 7) all arriving at once, through the server (admission,
     continuous batching, inputs made on the device at admission, results
     copied out at retirement) -> the most jobs/s it completes.  Beside it
-    the same jobs one after another on one stream, inputs made beforehand
-    and not timed: with a launch per kernel per step (pfs_fluid_timestep),
-    and with each job's steps replayed from hipGraphs (pfs_fluid_run).  The
-    server's dye results must equal the per-step loop's."""
+    the same jobs one after another on one stream, with the same work per
+    job (inputs made on the device, the steps, the result copied): with a
+    launch per kernel per step (pfs_fluid_timestep), and with each job's
+    steps replayed from hipGraphs (pfs_fluid_run).  The server's dye results
+    must equal the per-step loop's."""
     import torch
 
     from probabilistic_fluid_simulation_amd import fluid, scenes, serve
@@ -695,21 +696,24 @@ def served_capacity_leg(n_jobs: int, seed: int = 7) -> dict:
     srv.close()
 
     def loop(run):
-        bufs = [serve.device_scene(j) for j in jobs]
+        results = []
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for j, b in zip(jobs, bufs):
+        for j in jobs:
+            b = serve.device_scene(j)
             if run:
                 fluid.fluid_run(*b, j.dt, j.visc, j.iters, j.steps)
             else:
                 for _ in range(j.steps):
                     fluid.fluid_timestep(*b, j.dt, j.visc, j.iters)
+            results.append(b[2].data.clone())
+            del b
         torch.cuda.synchronize()
-        return time.perf_counter() - t0, bufs
+        return time.perf_counter() - t0, results
 
-    t_plain, bufs = loop(False)
-    same = all(torch.equal(j.result, b[2].data) for j, b in zip(jobs, bufs))
-    del bufs
+    t_plain, results = loop(False)
+    same = all(torch.equal(j.result, r) for j, r in zip(jobs, results))
+    del results
     t_run, _ = loop(True)
     cells = sum(j.fw * j.fh * j.steps for j in jobs)
     return {"jobs": n_jobs, "arrivals": "all at t = 0", "server_jobs_per_s": out["jobs_per_s"],
