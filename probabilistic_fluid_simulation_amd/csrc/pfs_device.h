@@ -97,21 +97,19 @@ constexpr unsigned kDivExpLo = 127 - 96, kDivExpHi = 127 + 95;
 __device__ __forceinline__ unsigned float_exp(float x) {
     return (__float_as_uint(x) >> 23) & 0xffu;
 }
+// One Markstein step: q + (x - q*beta)*y, each term rounded once.
+__device__ __forceinline__ float div_step(float q, float x, float beta, float y) {
+    return fmaf(fmaf(-q, beta, x), y, q);
+}
 __device__ __forceinline__ float div_fast(float x, float beta, float y) {
-    const float q0 = x * y;
-    const float r0 = fmaf(-q0, beta, x);
-    const float q1 = fmaf(r0, y, q0);
-    const float r1 = fmaf(-q1, beta, x);
-    return fmaf(r1, y, q1);
+    return div_step(div_step(x * y, x, beta, y), x, beta, y);
 }
 // When the host has checked |y*beta - 1| <= 2^-26 (exact in double), then
 // |x*y - x/beta| < ulp(x/beta)/4 for every x, so q0 = RN(x*y) is already
 // faithful and one Markstein step rounds correctly: 3 operations instead
 // of 5 (same range conditions on x).
 __device__ __forceinline__ float div_fast3(float x, float beta, float y) {
-    const float q0 = x * y;
-    const float r0 = fmaf(-q0, beta, x);
-    return fmaf(r0, y, q0);
+    return div_step(x * y, x, beta, y);
 }
 __device__ __forceinline__ float div_fixed(float x, float beta, float y, bool fast) {
     return (fast && div_fast_ok(x)) ? div_fast(x, beta, y) : x / beta;

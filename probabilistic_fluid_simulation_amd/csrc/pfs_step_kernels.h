@@ -294,18 +294,20 @@ __device__ __forceinline__ void diffuse_tile(const DiffuseArgs &a, int tile) {
         }
         // The range is tested before dividing, in the block that made the
         // numerators (the compiler then keeps min3/max3 on them).
+        // div_fast is div_fast3 followed by one more Markstein step: the
+        // step is applied in place, so both forms leave the quotients in the
+        // same registers (no copies on the 3-operation path).
         if (fast && lo >= 0x1p-96f && hi < 0x1p96f) {
-            if (short_div) {
+#pragma unroll
+            for (int k = 0; k < S; ++k) {
+                u[k] = div_fast3(nu[k], a.beta, a.rcp_beta);
+                v[k] = div_fast3(nv[k], a.beta, a.rcp_beta);
+            }
+            if (!short_div) {
 #pragma unroll
                 for (int k = 0; k < S; ++k) {
-                    u[k] = div_fast3(nu[k], a.beta, a.rcp_beta);
-                    v[k] = div_fast3(nv[k], a.beta, a.rcp_beta);
-                }
-            } else {
-#pragma unroll
-                for (int k = 0; k < S; ++k) {
-                    u[k] = div_fast(nu[k], a.beta, a.rcp_beta);
-                    v[k] = div_fast(nv[k], a.beta, a.rcp_beta);
+                    u[k] = div_step(u[k], nu[k], a.beta, a.rcp_beta);
+                    v[k] = div_step(v[k], nv[k], a.beta, a.rcp_beta);
                 }
             }
         } else {
@@ -375,15 +377,16 @@ __device__ __forceinline__ void pressure_tile(const PressureArgs &a, int tile) {
 
     // Pressure Jacobi sweeps (src/fluid.cpp:240-257):
     // p' = (pL + pR + pT + pB + 1*b) / 4, left to right.
-    float pp[S];  // p^(K-1)
-#pragma unroll 1
-    for (int s = 1; s <= NS; ++s) {
+    // Sweeps are unrolled (no copies of the iterate between them) and the
+    // last sweep of the last chunk is peeled, so that p^(K-1) is kept by
+    // plain register copies rather than a select in every sweep.
+    auto sweep = [&](int s) {
         float *B = plane + (s & 1) * PL;
 #pragma unroll
         for (int k = 0; k < S; ++k)
             if (st.active) B[(st.r0 + k) * RX + st.x] = p[k];
         __syncthreads();
-        if (!st.active) continue;
+        if (!st.active) return;
         const float top = B[(st.r0 > 0 ? st.r0 - 1 : 0) * RX + st.x];
         const float bot = B[(st.r0 + S < RY ? st.r0 + S : RY - 1) * RX + st.x];
         const float *Bl = B + st.r0 * RX + st.xl, *Br = B + st.r0 * RX + st.xr;
@@ -391,11 +394,18 @@ __device__ __forceinline__ void pressure_tile(const PressureArgs &a, int tile) {
 #pragma unroll
         for (int k = 0; k < S; ++k) {
             const float cur = p[k];
-            if (LAST && s == NS) pp[k] = cur;
             const float below = k == S - 1 ? bot : p[k + 1];
             p[k] = pressure_cell(Bl[k * RX], Br[k * RX], prev, below, rhs[k]);
             prev = cur;
         }
+    };
+    float pp[S];  // p^(K-1)
+#pragma unroll
+    for (int s = 1; s <= (LAST ? NS - 1 : NS); ++s) sweep(s);
+    if (LAST) {
+#pragma unroll
+        for (int k = 0; k < S; ++k) pp[k] = p[k];
+        sweep(NS);
     }
 
     if (!LAST) {
