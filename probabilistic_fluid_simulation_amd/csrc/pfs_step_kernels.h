@@ -27,8 +27,27 @@ struct Geo {
     static constexpr int RYP = SEGS * S;                // rows of an LDS plane (>= RY)
     static constexpr int NTH = RX * SEGS;               // threads owning cells
     static constexpr int NT = ((NTH + 63) / 64) * 64;   // whole wavefronts
+    // Row pitch of an FP32 LDS plane: the smallest P >= RX with S*P = RX
+    // (mod 64).  Thread t = seg*RX + x addresses row seg*S + k at
+    // (seg*S + k)*P + x, which is t + k*P (mod 64): a wavefront whose strips
+    // start in two segments still addresses 64 consecutive banks.  (The
+    // pressure planes use it; for the diffusion's float2 planes it removed
+    // conflicts without changing the kernel's time, so they keep pitch RX.)
+    static constexpr int pitch() {
+        for (int p = RX; p < RX + 64; ++p)
+            if ((S * p - RX) % 64 == 0) return p;
+        return RX;
+    }
+    static constexpr int P = pitch();
     static_assert(NTH <= NTMAX && RYP >= RY, "tile geometry");
 };
+
+// Workgroups of `threads` threads and `lds` bytes of LDS that one CU holds at
+// once (160 KiB of LDS, 32 wavefronts).
+constexpr int wgs_per_cu(int lds, int threads) {
+    const int by_lds = 163840 / lds, by_waves = 32 / ((threads + 63) / 64);
+    return by_lds < by_waves ? by_lds : by_waves;
+}
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
 // XCD b % 8).  Remap so that each XCD owns a contiguous band of tiles and the
@@ -350,7 +369,16 @@ __device__ __forceinline__ void diffuse_tile(const DiffuseArgs &a, int tile) {
 template <int H, bool LAST, int TS = kTile>
 __device__ __forceinline__ void pressure_tile(const PressureArgs &a, int tile) {
     using G = Geo<H, TS>;
-    constexpr int S = G::S, RX = G::RX, RY = G::RY, PL = G::RYP * RX;
+    constexpr int S = G::S, RY = G::RY;
+    // The padded pitch is taken only where it keeps the workgroups that fit
+    // a CU (160 KiB of LDS, 32 wavefronts): the planes plus the dye's
+    // velocity block.
+    constexpr int VN = LAST ? G::TX * G::TY * 8 : 8;
+    constexpr int PP = wgs_per_cu(2 * G::RYP * G::P * 4 + VN, G::NT) >=
+                               wgs_per_cu(2 * G::RYP * G::RX * 4 + VN, G::NT)
+                           ? G::P
+                           : G::RX;
+    constexpr int PL = G::RYP * PP;
     constexpr int NS = H - (LAST ? 1 : 0);
     __shared__ float plane[2 * PL];
 
@@ -384,18 +412,18 @@ __device__ __forceinline__ void pressure_tile(const PressureArgs &a, int tile) {
         float *B = plane + (s & 1) * PL;
 #pragma unroll
         for (int k = 0; k < S; ++k)
-            if (st.active) B[(st.r0 + k) * RX + st.x] = p[k];
+            if (st.active) B[(st.r0 + k) * PP + st.x] = p[k];
         __syncthreads();
         if (!st.active) return;
-        const float top = B[(st.r0 > 0 ? st.r0 - 1 : 0) * RX + st.x];
-        const float bot = B[(st.r0 + S < RY ? st.r0 + S : RY - 1) * RX + st.x];
-        const float *Bl = B + st.r0 * RX + st.xl, *Br = B + st.r0 * RX + st.xr;
+        const float top = B[(st.r0 > 0 ? st.r0 - 1 : 0) * PP + st.x];
+        const float bot = B[(st.r0 + S < RY ? st.r0 + S : RY - 1) * PP + st.x];
+        const float *Bl = B + st.r0 * PP + st.xl, *Br = B + st.r0 * PP + st.xr;
         float prev = top;
 #pragma unroll
         for (int k = 0; k < S; ++k) {
             const float cur = p[k];
             const float below = k == S - 1 ? bot : p[k + 1];
-            p[k] = pressure_cell(Bl[k * RX], Br[k * RX], prev, below, rhs[k]);
+            p[k] = pressure_cell(Bl[k * PP], Br[k * PP], prev, below, rhs[k]);
             prev = cur;
         }
     };
@@ -419,7 +447,7 @@ __device__ __forceinline__ void pressure_tile(const PressureArgs &a, int tile) {
     float *B = plane + ((NS + 1) & 1) * PL;
 #pragma unroll
     for (int k = 0; k < S; ++k)
-        if (st.active) B[(st.r0 + k) * RX + st.x] = p[k];
+        if (st.active) B[(st.r0 + k) * PP + st.x] = p[k];
     __syncthreads();
     // The new velocities also go to LDS as they are made, for the dye pass.
     __shared__ float2 Vn[LAST ? G::TX * G::TY : 1];
@@ -427,9 +455,9 @@ __device__ __forceinline__ void pressure_tile(const PressureArgs &a, int tile) {
 #pragma unroll
     for (int k = 0; k < S; ++k) {
         if (!st.mine(k, a.w, a.h)) continue;
-        const int o = (st.r0 + k) * RX + st.x;
+        const int o = (st.r0 + k) * PP + st.x;
         const float gradX = (B[o + 1] - B[o - 1]) * a.dt / 2.0f;
-        const float gradY = (B[o + RX] - B[o - RX]) * a.dt / 2.0f;
+        const float gradY = (B[o + PP] - B[o - PP]) * a.dt / 2.0f;
         const int cell = st.cell(k, a.w);
         const float un = e[k].x - gradX, vn = e[k].y - gradY;
         P4[cell] = make_float4(un, vn, pp[k], rhs[k]);
