@@ -207,23 +207,13 @@ __device__ __forceinline__ void diffuse_tile(const DiffuseArgs &a, int tile) {
     const Strip<G> st(threadIdx.x, tile, a.tiles_x, a.w, a.h);
     const float2 *__restrict__ P2 = reinterpret_cast<const float2 *>(a.P);  // (u,v) at 2*cell
 
-    // Every global load of the launch is issued here: the staged region, and
-    // (last chunk) the tile cells' pressure start, whose latency then hides
-    // behind the advection and the sweeps.
+    // The staged region, loaded at entry: the advection needs all of it.
     float2 c[S];
 #pragma unroll
     for (int k = 0; k < S; ++k) {
         const int g = st.row_ok(k) ? st.gj[k] * a.w + st.gi : 0;
         c[k] = a.first ? P2[2 * g] : a.dsrc[g];
     }
-    float warm[S];
-    if (LAST) {
-        const float *__restrict__ W = a.k_even ? a.Q : a.P;
-#pragma unroll
-        for (int k = 0; k < S; ++k)  // other rows load cell 0, unused
-            warm[k] = W[4 * (st.mine(k, a.w, a.h) ? st.cell(k, a.w) : 0) + 2];
-    }
-
     float u[S], v[S];
     if (a.first) {
         // Advect every strip cell (src/fluid.cpp:36-67) with its corners
@@ -262,6 +252,18 @@ __device__ __forceinline__ void diffuse_tile(const DiffuseArgs &a, int tile) {
             u[k] = c[k].x;
             v[k] = c[k].y;
         }
+    }
+
+    // The tile cells' pressure start (last chunk), needed only at the end:
+    // issued after the advection, so that these loads (a whole 16-byte cell
+    // for 4 bytes when they come from tmp) stream in under the sweeps instead
+    // of competing with the staged region at entry.
+    float warm[S];
+    if (LAST) {
+        const float *__restrict__ W = a.k_even ? a.Q : a.P;
+#pragma unroll
+        for (int k = 0; k < S; ++k)  // other rows load cell 0, unused
+            warm[k] = W[4 * (st.mine(k, a.w, a.h) ? st.cell(k, a.w) : 0) + 2];
     }
 
     // Jacobi smoothing sweeps (src/fluid.cpp:154-185):
@@ -386,7 +388,9 @@ __device__ __forceinline__ void pressure_tile(const PressureArgs &a, int tile) {
     float4 *__restrict__ P4 = reinterpret_cast<float4 *>(a.P);
     float4 *__restrict__ Q4 = reinterpret_cast<float4 *>(a.Q);
 
-    // All global loads up front (see k_diffuse).
+    // The iterate and right-hand side of the staged region, needed by the
+    // first sweep.  E, needed only by the gradient, is loaded after the first
+    // sweep, under the arithmetic of the rest.
     float p[S], rhs[S];
     {
         const float *__restrict__ src = a.first ? a.pw : a.psrc;
@@ -397,12 +401,6 @@ __device__ __forceinline__ void pressure_tile(const PressureArgs &a, int tile) {
             rhs[k] = a.bpl[g];
         }
     }
-    float2 e[S];  // E of the tile cells (other rows load cell 0, unused)
-    if (LAST) {
-#pragma unroll
-        for (int k = 0; k < S; ++k) e[k] = a.E[st.mine(k, a.w, a.h) ? st.cell(k, a.w) : 0];
-    }
-
     // Pressure Jacobi sweeps (src/fluid.cpp:240-257):
     // p' = (pL + pR + pT + pB + 1*b) / 4, left to right.
     // Sweeps are unrolled (no copies of the iterate between them) and the
@@ -427,10 +425,20 @@ __device__ __forceinline__ void pressure_tile(const PressureArgs &a, int tile) {
             prev = cur;
         }
     };
-    float pp[S];  // p^(K-1)
+    float2 e[S];  // E of the tile cells (other rows load cell 0, unused)
+    auto load_e = [&]() {
 #pragma unroll
-    for (int s = 1; s <= (LAST ? NS - 1 : NS); ++s) sweep(s);
+        for (int k = 0; k < S; ++k) e[k] = a.E[st.mine(k, a.w, a.h) ? st.cell(k, a.w) : 0];
+    };
+    float pp[S];  // p^(K-1)
+    constexpr int NLOOP = LAST ? NS - 1 : NS;
+#pragma unroll
+    for (int s = 1; s <= NLOOP; ++s) {
+        sweep(s);
+        if (LAST && s == 1) load_e();
+    }
     if (LAST) {
+        if (NLOOP < 1) load_e();
 #pragma unroll
         for (int k = 0; k < S; ++k) pp[k] = p[k];
         sweep(NS);
