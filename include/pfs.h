@@ -208,8 +208,11 @@ int pfs_profile_step(pfs_field *vp, pfs_field *tmp, pfs_field *image, pfs_field 
  * No counterpart in the reference, which steps one scene per call.  A batch
  * holds many independent scenes (caller-owned device buffers, as for
  * pfs_fluid_timestep) and advances all of them by one timestep each with one
- * launch per kernel stage; the launch sequence is replayed from a hipGraph
- * while membership is unchanged.  Every scene's results are bit-identical to
+ * launch per kernel stage; the launch sequences of 8 and of 2 steps are
+ * replayed from hipGraphs while membership is unchanged (in them each step's
+ * separate dye pass runs beside the next step's diffusion kernel; the batch
+ * makes one more non-blocking stream of default priority for capturing that
+ * branch).  Every scene's results are bit-identical to
  * stepping it alone with pfs_fluid_timestep.  Scenes need 4-channel,
  * 16-byte aligned velocity fields and 1 <= iters <= 512; the dye may have any
  * size and channel count. */

@@ -15,9 +15,13 @@
 // Each workgroup finds its scene through a tile -> scene table and runs the
 // same per-tile body as the single-scene kernels (pfs_step_kernels.h), so
 // every scene's result is bit-identical to stepping it alone.  The launch
-// sequence of two consecutive steps (the dye buffers swap every step) is
-// captured once in a hipGraph and replayed while membership is unchanged;
-// adding or removing a scene rebuilds the tables and the graph.
+// sequences of 8 and of 2 consecutive steps (the dye buffers swap every step)
+// are captured once in hipGraphs and replayed while membership is unchanged;
+// adding or removing a scene rebuilds the tables and the graphs.  In a
+// captured run the dye pass of step k is a branch of its own: it reads the
+// velocity that step k's pressure kernels wrote and writes only the dye, so
+// it runs beside step k+1's diffusion kernels (which only read the
+// velocity); step k+1's pressure kernels, which overwrite it, wait for it.
 
 #include <hip/hip_runtime.h>
 
@@ -152,7 +156,11 @@ struct pfs_batch {
     int *block_scene = nullptr;
     int nblocks = 0;
     void *dev_tables = nullptr;
-    hipGraphExec_t graph = nullptr;
+    static constexpr int kRunSteps = 8;  // steps of the longer captured run (even)
+    hipGraphExec_t graph = nullptr;      // 2 steps
+    hipGraphExec_t graph_run = nullptr;  // kRunSteps steps
+    hipStream_t side = nullptr;          // the dye branch while capturing
+    hipEvent_t fork_ev[kRunSteps] = {}, join_ev[kRunSteps] = {};
     long long t = 0;        // steps enqueued so far
     long long t_build = 0;  // t when the tables were built
     std::vector<Retired> retired;
@@ -196,8 +204,10 @@ struct pfs_batch {
 
     int drop_tables() {
         PFS_TRY(retire(dev_tables, graph));
+        PFS_TRY(retire(nullptr, graph_run));
         dev_tables = nullptr;
         graph = nullptr;
+        graph_run = nullptr;
         groups.clear();
         ctab[0] = ctab[1] = nullptr;
         block_scene = nullptr;
@@ -368,11 +378,70 @@ struct pfs_batch {
         return PFS_OK;
     }
 
-    int capture() {
+    // The side stream and events of a captured run (made once).
+    int capture_resources() {
+        if (side) return PFS_OK;
+        for (int k = 0; k < kRunSteps; ++k) {
+            PFS_HIP_TRY(hipEventCreateWithFlags(&fork_ev[k], hipEventDisableTiming));
+            PFS_HIP_TRY(hipEventCreateWithFlags(&join_ev[k], hipEventDisableTiming));
+        }
+        PFS_HIP_TRY(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        return PFS_OK;
+    }
+
+    void free_capture_resources() {
+        for (int k = 0; k < kRunSteps; ++k) {
+            if (fork_ev[k]) (void)hipEventDestroy(fork_ev[k]);
+            if (join_ev[k]) (void)hipEventDestroy(join_ev[k]);
+            fork_ev[k] = join_ev[k] = nullptr;
+        }
+        if (side) (void)hipStreamDestroy(side);
+        side = nullptr;
+    }
+
+    // n steps (n even, table parity k & 1 for step k), the dye pass of each
+    // on the side stream: forked after the step's pressure kernels, joined
+    // before the next step's pressure kernels and at the end.
+    int enqueue_run(int n) {
+        for (int k = 0; k < n; ++k) {
+            const int q = k & 1;
+            bool joined = k == 0 || nblocks == 0;
+            for (const Group &g : groups) {
+                for (int c = 0; c < g.nchunks; ++c) {
+                    hipLaunchKernelGGL(diffuse_batch_for(g.plan[c].H, g.plan[c].last, tile),
+                                       dim3(g.ntiles), dim3(threads_for(g.plan[c].H, tile)), 0,
+                                       stream, g.dtab[c], g.tile_scene, g.ntiles);
+                    PFS_TRY(check_launch("batched diffuse"));
+                }
+                if (!joined) {
+                    PFS_HIP_TRY(hipStreamWaitEvent(stream, join_ev[k - 1], 0));
+                    joined = true;
+                }
+                for (int c = 0; c < g.nchunks; ++c) {
+                    hipLaunchKernelGGL(pressure_batch_for(g.plan[c].H, g.plan[c].last, tile),
+                                       dim3(g.ntiles), dim3(threads_for(g.plan[c].H, tile)), 0,
+                                       stream, g.ptab[q][c], g.tile_scene, g.ntiles);
+                    PFS_TRY(check_launch("batched pressure"));
+                }
+            }
+            if (nblocks) {
+                PFS_HIP_TRY(hipEventRecord(fork_ev[k], stream));
+                PFS_HIP_TRY(hipStreamWaitEvent(side, fork_ev[k], 0));
+                hipLaunchKernelGGL(k_color_batch, dim3(nblocks), dim3(kCBX * kCBY), 0, side, ctab[q],
+                                   block_scene);
+                PFS_TRY(check_launch("batched dye"));
+                PFS_HIP_TRY(hipEventRecord(join_ev[k], side));
+            }
+        }
+        if (nblocks) PFS_HIP_TRY(hipStreamWaitEvent(stream, join_ev[n - 1], 0));
+        return PFS_OK;
+    }
+
+    int capture(int n, hipGraphExec_t *out) {
+        PFS_TRY(capture_resources());
         hipGraph_t g = nullptr;
         PFS_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_step(0);
-        if (rc == PFS_OK) rc = enqueue_step(1);
+        int rc = enqueue_run(n);
         hipError_t e = hipStreamEndCapture(stream, &g);
         if (rc != PFS_OK) {
             if (g) (void)hipGraphDestroy(g);
@@ -380,12 +449,21 @@ struct pfs_batch {
         }
         if (e != hipSuccess)
             return set_error(PFS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-        e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+        e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         if (e != hipSuccess) {
-            graph = nullptr;
+            *out = nullptr;
             return set_error(PFS_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
         }
+        return PFS_OK;
+    }
+
+    // Captures the graphs the next steps would capture on first use.
+    int prepare_graphs() {
+        if (dirty) PFS_TRY(rebuild());
+        if (!use_graph) return PFS_OK;
+        if (!graph) PFS_TRY(capture(2, &graph));
+        if (!graph_run) PFS_TRY(capture(kRunSteps, &graph_run));
         return PFS_OK;
     }
 
@@ -397,8 +475,13 @@ struct pfs_batch {
         if (!any) return PFS_OK;
         for (int left = n; left > 0;) {
             const int q = (int)((t - t_build) & 1);
-            if (use_graph && q == 0 && left >= 2) {
-                if (!graph) PFS_TRY(capture());
+            if (use_graph && q == 0 && left >= kRunSteps) {
+                if (!graph_run) PFS_TRY(capture(kRunSteps, &graph_run));
+                PFS_HIP_TRY(hipGraphLaunch(graph_run, stream));
+                t += kRunSteps;
+                left -= kRunSteps;
+            } else if (use_graph && q == 0 && left >= 2) {
+                if (!graph) PFS_TRY(capture(2, &graph));
                 PFS_HIP_TRY(hipGraphLaunch(graph, stream));
                 t += 2;
                 left -= 2;
@@ -436,6 +519,7 @@ int pfs_batch_destroy(pfs_batch *b) {
     if (rc == PFS_OK) rc = b->sync();
     for (auto &s : b->scenes)
         if (s.ws) (void)hipFree(s.ws);
+    b->free_capture_resources();
     if (b->own_stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return rc;
@@ -610,7 +694,8 @@ int pfs_batch_autotune(pfs_batch *b, int reps, int *side, float *ms, int n_ms) {
     for (int k = 0; k < 3 && rc == PFS_OK; ++k) {
         b->tile_forced = kTileSides[k];
         b->dirty = true;
-        rc = b->step(2);  // untimed: tables, graph, first launches
+        rc = b->step(2);  // untimed: tables, graphs, first launches
+        if (rc == PFS_OK) rc = b->prepare_graphs();
         if (rc == PFS_OK && hipEventRecord(ev[0], b->stream) != hipSuccess)
             rc = set_error(PFS_ERR_HIP, "autotune: event");
         if (rc == PFS_OK) rc = b->step(2 * reps);

@@ -48,9 +48,9 @@ def test_batch_mixed_scenes(orc, graphs, tile):
     assert b.count == len(MIX)
     b.set_tile(tile)
     assert b.tile == (tile or 16)  # these scenes hold fewer than 128 tiles of 32
-    b.step(5)
+    b.step(11)  # with graphs: a captured run of 8 steps, one of 2, one step launched
     for slot, s in zip(slots, sc):
-        _check(b, slot, s, orc, 5)
+        _check(b, slot, s, orc, 11)
     b.close()
 
 

@@ -71,11 +71,13 @@ def _oracle_end(orc, bufs, job, steps):
 
 def test_batched_form_256_scenes(orc):
     """The batched workload's 256 scenes (dye 4K-256K px, field 64^2-512^2,
-    scene s from seed s, K = 8) in one Batch for 3 steps; 16 scenes, four of
-    each field size, end bit-identical to the oracle stepping them alone."""
+    scene s from seed s, K = 8) in one Batch for 10 steps (a captured run of
+    8, whose dye passes overlap the next steps' diffusion, then one of 2);
+    16 scenes, four of each field size, end bit-identical to the oracle
+    stepping them alone."""
     specs = scenes.batched_specs(256)
     jobs = [serve.Job(job=sp["scene"], arrival=0.0, fw=sp["fw"], fh=sp["fh"], iw=sp["iw"],
-                      ih=sp["ih"], steps=3, seed=sp["seed"], iters=8) for sp in specs]
+                      ih=sp["ih"], steps=10, seed=sp["seed"], iters=8) for sp in specs]
     sample = []
     for side in (64, 128, 256, 512):
         sample += [j for j in jobs if j.fw == side][:4]
@@ -89,10 +91,10 @@ def test_batched_form_256_scenes(orc):
             inputs[j.job] = [t.data.cpu().numpy().copy() for t in bufs]
         slots[j.job] = b.add(*bufs, j.dt, j.visc, j.iters)
     assert b.count == 256
-    b.step(3)
+    b.step(10)
     b.sync()
     for j in sample:
-        want = orc.run(*[a.copy() for a in inputs[j.job]], j.dt, j.visc, j.iters, 3)
+        want = orc.run(*[a.copy() for a in inputs[j.job]], j.dt, j.visc, j.iters, 10)
         got = [t.cpu().numpy() for t in b.buffers(slots[j.job])]
         for name, g, w in zip(("vp", "tmp", "image", "itmp"), got, want):
             assert g.tobytes() == w.tobytes(), (j.job, j.fw, j.iw, name)
