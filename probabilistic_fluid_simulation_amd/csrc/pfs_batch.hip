@@ -378,14 +378,22 @@ struct pfs_batch {
         return PFS_OK;
     }
 
-    // The side stream and events of a captured run (made once).
+    // The side stream and events of a captured run (made once; on failure
+    // nothing is kept and the next capture tries again).
     int capture_resources() {
         if (side) return PFS_OK;
-        for (int k = 0; k < kRunSteps; ++k) {
-            PFS_HIP_TRY(hipEventCreateWithFlags(&fork_ev[k], hipEventDisableTiming));
-            PFS_HIP_TRY(hipEventCreateWithFlags(&join_ev[k], hipEventDisableTiming));
+        hipError_t e = hipSuccess;
+        for (int k = 0; k < kRunSteps && e == hipSuccess; ++k) {
+            e = hipEventCreateWithFlags(&fork_ev[k], hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&join_ev[k], hipEventDisableTiming);
         }
-        PFS_HIP_TRY(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&side, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            side = nullptr;
+            free_capture_resources();
+            return set_error(PFS_ERR_HIP, std::string("batch capture resources: ") + hipGetErrorString(e));
+        }
         return PFS_OK;
     }
 
