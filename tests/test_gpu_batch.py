@@ -144,9 +144,13 @@ def test_batch_workspace_and_errors():
     np.testing.assert_equal(vp.data.shape, (64, 64, 4))
 
 
-def test_server_continuous_batching_matches_alone():
+@pytest.mark.parametrize("substeps", [4, 16])
+def test_server_continuous_batching_matches_alone(orc, substeps):
     """Jobs arriving over time, admitted into a running batch and retired as
-    they finish, give the same dye as each job run alone."""
+    they finish, give the same dye as each job run alone; with runs of up to
+    16 steps between admission points the batch replays its 8-step graphs
+    (dye passes beside the next steps' diffusion), and every job's dye also
+    equals the oracle's."""
     from probabilistic_fluid_simulation_amd import serve
 
     specs = scenes.served_specs(12, rate=400.0, seed=7)
@@ -155,7 +159,7 @@ def test_server_continuous_batching_matches_alone():
         sp.update(fw=side, fh=side, iw=side, ih=side, steps=6 + sp["steps"] % 35)
     specs[3].update(iw=specs[3]["fw"] + 5)  # one dye of another size: separate dye kernel
     jobs = serve.jobs_from_specs(specs, iters=4)
-    srv = serve.Server(max_substeps=4, keep_results=True)
+    srv = serve.Server(max_substeps=substeps, keep_results=True)
     out = srv.run(jobs)
     srv.close()
     assert out["completed"] == len(jobs)
@@ -163,11 +167,16 @@ def test_server_continuous_batching_matches_alone():
     for j in jobs:
         alone = Batch()
         bufs = serve.device_scene(j)
+        if substeps > 4:
+            host = [t.data.cpu().numpy().copy() for t in bufs]
         slot = alone.add(*bufs, j.dt, j.visc, j.iters)
         alone.step(j.steps)
         alone.sync()
         assert torch.equal(alone.image(slot), j.result), j.job
         alone.close()
+        if substeps > 4:
+            want = orc.run(*host, j.dt, j.visc, j.iters, j.steps)
+            assert j.result.cpu().numpy().tobytes() == want[2].tobytes(), j.job
 
 
 def test_server_admission_respects_budget():
