@@ -262,6 +262,41 @@ def test_steps_large_backtraces(orc):
     assert_bitexact(gpu_run(s, 2), oracle_run(orc, s, 2))
 
 
+def test_steps_non_finite_inputs():
+    """NaN and infinite velocities and dye values: the reference indexes
+    memory out of bounds for them (src/fluid.cpp:55-57 casts a NaN
+    back-trace to int), so there is no parity to check; here every path
+    clamps the corner keys into the field, runs to the end, and the fused
+    timestep, the two calls and the per-operator loop agree on every value
+    (NaNs compared by position)."""
+    s = scenes.random_scene(96, 80, 96, 80, seed=21, dt=0.5, visc=0.1, iters=4)
+    s.vp[10, 20, 0] = np.nan
+    s.vp[50, 60, 1] = np.inf
+    s.vp[70, 5, 0] = -np.inf
+    s.image[30, 40, 2] = np.nan
+    s.image[31, 41, 0] = np.inf
+
+    def canon(bufs):
+        out = []
+        for a in bufs:
+            nan = np.isnan(a)
+            out.append((nan, np.where(nan, np.float32(0), a).view(np.uint32)))
+        return out
+
+    runs = [gpu_run(s, 2, combined=True), gpu_run(s, 2)]
+    fluid.set_step_path("per_op")
+    try:
+        runs.append(gpu_run(s, 2))
+    finally:
+        fluid.set_step_path("auto")
+    ref = canon(runs[0])
+    assert any(m.any() for m, _ in ref), "the NaNs did not propagate"
+    for run in runs[1:]:
+        for (m0, b0), (m1, b1), name in zip(ref, canon(run), ("vp", "tmp", "image", "itmp")):
+            np.testing.assert_array_equal(m0, m1, err_msg=name)
+            np.testing.assert_array_equal(b0, b1, err_msg=name)
+
+
 def _gold_cases():
     with np.load(os.path.join(GOLD, "steps_small.npz")) as z:
         n = len([k for k in z.files if k.endswith("_meta")])
