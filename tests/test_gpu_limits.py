@@ -48,6 +48,12 @@ def _bits(t):
 @pytest.mark.parametrize("w,h", SHAPES, ids=[f"{w}x{h}" for w, h in SHAPES])
 def test_largest_fields_fixed_point(w, h):
     assert w * h * 4 <= 2**31 - 1 < (w + 1) * h * 4
+    # vp, tmp, image, itmp, copies of vp and image, the step's workspace
+    # (32 B per cell) and the pattern's temporaries: about 10 buffers' worth.
+    need = 10 * w * h * 16
+    free, _ = torch.cuda.mem_get_info()
+    if free < need:
+        pytest.skip(f"needs {need / 2**30:.0f} GiB of free device memory, {free / 2**30:.0f} free")
     vp, img = _fixed_point(w, h)
     vp0, img0 = vp.clone(), img.clone()
     dt, visc, k = 1.0 / 120.0, 0.1, 8
